@@ -1,0 +1,682 @@
+// C-ABI implementation: workspaces, the shrink-wrap driver and the
+// per-operator entry points.  See include/imresh_hip.h for the interface.
+//
+// Driver (one reconstruction, all on one HIP stream, no host round trip
+// until the end -- the reference synchronized twice per cycle for the max
+// and the error, cudaShrinkWrap.cu:274,368):
+//
+//   COL_INIT  : M -> inverse column FFT                  (cudaShrinkWrap.cu:229-232)
+//   ROW_INIT  : inverse row FFT -> g' , gPrev = g', |g'|  (:232, :304-309, :240)
+//   per cycle c:
+//     blur2d       |g'| -> blurred + tile max              (:242, :274)
+//     max_finalize threshold = cut_c * max                 (:275-276)
+//     HIO loop:    ROW_FIRST_CUT (mask cut-off, :277 + constraint :314) or
+//                  ROW_MID (inverse rows, constraint) ; COL_PROJ (FFT, modulus, IFFT)
+//     ROW_LAST     inverse rows -> g', |g'| for the next blur, error sums (:368)
+//     err_finalize error, early-exit flag (:374-375)
+//   copy_from_real  Re(g') -> io                           (:401)
+//
+// The early exit sets a device flag that turns every later kernel of the
+// reconstruction into a no-op, so the host never waits inside the loop.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace imr;
+
+namespace {
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Workspace {
+    int device = 0;
+    int W = 0, H = 0;
+    size_t n = 0;
+    void *base = nullptr;
+    float2 *X = nullptr, *G = nullptr;
+    float *Bn = nullptr, *B2 = nullptr, *M = nullptr;
+    uint8_t *mask = nullptr;
+    float *pmax = nullptr;
+    float *perr = nullptr;
+    unsigned *pcnt = nullptr;
+    float *scal = nullptr;      // [0] threshold, [1..3] error, total, nmasked
+    int *flags = nullptr;       // [0] done, [1] cycles
+    float *weights = nullptr;   // [cycles][64]
+    unsigned wcap = 0;
+    std::vector<int> taps;
+    float w_sigma0 = -1.f, w_dsigma = -1.f;
+    unsigned w_ncycles = 0;
+    float *host_stage = nullptr;  // pinned, 64 floats + readback
+    hipEvent_t ready = nullptr;   // recorded when the last user's work is queued
+    bool pending = false;
+};
+
+std::mutex g_ws_mtx;
+std::map<std::tuple<int, int, int>, std::vector<Workspace *>> g_ws_free;
+
+int ws_alloc(int device, int W, int H, Workspace **out)
+{
+    Workspace *w = new Workspace();
+    w->device = device;
+    w->W = W;
+    w->H = H;
+    w->n = (size_t) W * H;
+    const size_t n = w->n;
+    const int nblur = blur_grid(W, H);
+    const int nrow = H;   // >= number of row workgroups
+    size_t off = 0;
+    const size_t oX = off; off = align256(off + 8 * n);
+    const size_t oG = off; off = align256(off + 8 * n);
+    const size_t oBn = off; off = align256(off + 4 * n);
+    const size_t oB2 = off; off = align256(off + 4 * n);
+    const size_t oM = off; off = align256(off + 4 * n);
+    const size_t oMask = off; off = align256(off + n);
+    const size_t oPmax = off; off = align256(off + 4 * (size_t) nblur);
+    const size_t oPerr = off; off = align256(off + 4 * (size_t) nrow);
+    const size_t oPcnt = off; off = align256(off + 4 * (size_t) nrow);
+    const size_t oScal = off; off = align256(off + 64);
+    const size_t oFlags = off; off = align256(off + 64);
+    hipError_t e = hipMalloc(&w->base, off);
+    if (e != hipSuccess) {
+        delete w;
+        return from_hip(e);
+    }
+    char *b = (char *) w->base;
+    w->X = (float2 *) (b + oX);
+    w->G = (float2 *) (b + oG);
+    w->Bn = (float *) (b + oBn);
+    w->B2 = (float *) (b + oB2);
+    w->M = (float *) (b + oM);
+    w->mask = (uint8_t *) (b + oMask);
+    w->pmax = (float *) (b + oPmax);
+    w->perr = (float *) (b + oPerr);
+    w->pcnt = (unsigned *) (b + oPcnt);
+    w->scal = (float *) (b + oScal);
+    w->flags = (int *) (b + oFlags);
+    e = hipHostMalloc((void **) &w->host_stage, 4096, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&w->ready, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        hipFree(w->base);
+        if (w->host_stage) hipHostFree(w->host_stage);
+        delete w;
+        return from_hip(e);
+    }
+    *out = w;
+    return IMRESH_OK;
+}
+
+// Workspaces are reused in stream order: a checked-out workspace makes the
+// new stream wait for the previous user's work (no host synchronization).
+int ws_checkout(int device, int W, int H, hipStream_t s, Workspace **out)
+{
+    Workspace *w = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mtx);
+        auto &v = g_ws_free[{device, W, H}];
+        if (!v.empty()) {
+            w = v.back();
+            v.pop_back();
+        }
+    }
+    if (w == nullptr) return ws_alloc(device, W, H, out);
+    if (w->pending) IMR_HIP_CHECK(hipStreamWaitEvent(s, w->ready, 0));
+    *out = w;
+    return IMRESH_OK;
+}
+
+void ws_return(Workspace *w, hipStream_t s)
+{
+    w->pending = hipEventRecord(w->ready, s) == hipSuccess;
+    if (!w->pending) hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lk(g_ws_mtx);
+    g_ws_free[{w->device, w->W, w->H}].push_back(w);
+}
+
+struct Params {
+    unsigned n_cycles, n_hio;
+    float target, beta, cutA, cut, sigma0, dsigma;
+};
+
+int normalize(const imresh_params *p, Params &q)
+{
+    imresh_params d;
+    imresh_params_default(&d);
+    if (p == nullptr) p = &d;
+    const bool queue = p->defaults == IMRESH_DEFAULTS_QUEUE;
+    q.n_cycles = p->n_cycles;
+    if (queue && q.n_cycles == 0) q.n_cycles = 20;
+    q.target = p->target_error <= 0 ? 1e-5f : p->target_error;
+    q.beta = p->hio_beta <= 0 ? 0.9f : p->hio_beta;
+    q.cutA = p->cutoff_autocorel <= 0 ? (queue ? 0.02f : 0.04f) : p->cutoff_autocorel;
+    q.cut = p->cutoff <= 0 ? 0.2f : p->cutoff;
+    q.sigma0 = p->sigma0 <= 0 ? 3.0f : p->sigma0;
+    q.dsigma = p->sigma_change <= 0 ? 0.01f : p->sigma_change;
+    q.n_hio = p->n_hio_cycles == 0 ? 20 : p->n_hio_cycles;
+    return IMRESH_OK;
+}
+
+// per-cycle Gaussian weights, uploaded once per parameter set
+int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
+{
+    if (w->weights && w->w_sigma0 == q.sigma0 && w->w_dsigma == q.dsigma && w->w_ncycles == q.n_cycles)
+        return IMRESH_OK;
+    std::vector<float> sig;
+    sigma_schedule(q.sigma0, q.dsigma, q.n_cycles, sig);
+    std::vector<float> host((size_t) std::max(1u, q.n_cycles) * 64, 0.f);
+    w->taps.assign(q.n_cycles, 0);
+    for (unsigned c = 0; c < q.n_cycles; ++c) {
+        const int k = gaussian_weights(sig[c], nullptr, 0);
+        if (k > 63) return IMRESH_ERR_KERNEL_TOO_LARGE;
+        gaussian_weights(sig[c], &host[(size_t) c * 64], 64);
+        w->taps[c] = k;
+    }
+    if (w->wcap < std::max(1u, q.n_cycles)) {
+        if (w->weights) {
+            IMR_HIP_CHECK(hipStreamSynchronize(s));
+            hipFree(w->weights);
+            w->weights = nullptr;
+        }
+        w->wcap = std::max(1u, q.n_cycles);
+        IMR_HIP_CHECK(hipMalloc(&w->weights, sizeof(float) * 64 * w->wcap));
+    }
+    IMR_HIP_CHECK(hipMemcpyAsync(w->weights, host.data(), sizeof(float) * host.size(),
+                                 hipMemcpyHostToDevice, s));
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    w->w_sigma0 = q.sigma0;
+    w->w_dsigma = q.dsigma;
+    w->w_ncycles = q.n_cycles;
+    return IMRESH_OK;
+}
+
+int run_reconstruction(Workspace *w, float *io_dev, const Params &q, hipStream_t s)
+{
+    Plan2D p;
+    IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
+    IMR_CHECK(ensure_weights(w, q, s));
+    IMR_HIP_CHECK(hipMemsetAsync(w->flags, 0, 64, s));
+    IMR_HIP_CHECK(hipMemsetAsync(w->scal, 0, 64, s));
+    const size_t n = w->n;
+
+    ColArgs ca{};
+    ca.X = w->X;
+    ca.M = io_dev;
+    ca.done = w->flags;
+    RowArgs ra{};
+    ra.X = w->X;
+    ra.G = w->G;
+    ra.mask = w->mask;
+    ra.Bn = w->Bn;
+    ra.B2 = w->B2;
+    ra.thr = w->scal;
+    ra.beta = q.beta;
+    ra.perr = w->perr;
+    ra.pcnt = w->pcnt;
+    ra.done = w->flags;
+
+    IMR_CHECK(launch_col(COL_INIT, p, ca, s));
+    IMR_CHECK(launch_row(ROW_INIT, p, ra, s));
+    const int nrowwg = row_grid(p);
+    const int nblur = blur_grid(w->W, w->H);
+    for (unsigned c = 0; c < q.n_cycles; ++c) {
+        BlurArgs ba{};
+        ba.in = w->Bn;
+        ba.out = w->B2;
+        ba.wh = ba.wv = w->weights + (size_t) c * 64;
+        ba.rh = ba.rv = (w->taps[c] - 1) / 2;
+        ba.W = w->W;
+        ba.H = w->H;
+        ba.pmax = w->pmax;
+        ba.done = w->flags;
+        IMR_CHECK(launch_blur(ba, s));
+        IMR_CHECK(launch_max_finalize(w->pmax, nblur, c == 0 ? q.cutA : q.cut, w->scal, w->flags, s));
+        for (unsigned it = 0; it < q.n_hio; ++it) {
+            IMR_CHECK(launch_row(it == 0 ? ROW_FIRST_CUT : ROW_MID, p, ra, s));
+            IMR_CHECK(launch_col(COL_PROJ, p, ca, s));
+        }
+        IMR_CHECK(launch_row(ROW_LAST, p, ra, s));
+        IMR_CHECK(launch_err_finalize(w->perr, w->pcnt, nrowwg, q.target, w->scal + 1, w->flags,
+                                      (unsigned *) (w->flags + 1), s));
+    }
+    IMR_CHECK(launch_copy_from_real(io_dev, w->X, n, s));
+    return IMRESH_OK;
+}
+
+int read_result(Workspace *w, hipStream_t s, imresh_result *res)
+{
+    float *st = w->host_stage;
+    IMR_HIP_CHECK(hipMemcpyAsync(st, w->scal, 16, hipMemcpyDeviceToHost, s));
+    IMR_HIP_CHECK(hipMemcpyAsync(st + 8, w->flags, 8, hipMemcpyDeviceToHost, s));
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    int flags[2];
+    std::memcpy(flags, st + 8, 8);
+    res->cycles_run = (uint32_t) flags[1];
+    res->last_error = st[1];
+    res->last_total = st[2];
+    res->last_masked = st[3];
+    return IMRESH_OK;
+}
+
+int current_device(int *dev)
+{
+    hipError_t e = hipGetDevice(dev);
+    return from_hip(e);
+}
+
+hipStream_t S(void *s) { return (hipStream_t) s; }
+
+// stream-ordered scratch for the per-operator reductions
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s;
+    explicit Scratch(hipStream_t st) : s(st) {}
+    int alloc(size_t bytes) { return from_hip(hipMallocAsync(&p, bytes, s)); }
+    ~Scratch()
+    {
+        if (p) hipFreeAsync(p, s);
+    }
+};
+
+int check_n(const void *p, uint32_t n)
+{
+    if (p == nullptr) return IMRESH_ERR_NULL;
+    (void) n;
+    return IMRESH_OK;
+}
+
+int read_scalar(const float *dev, float *host, hipStream_t s)
+{
+    IMR_HIP_CHECK(hipMemcpyAsync(host, dev, sizeof(float), hipMemcpyDeviceToHost, s));
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    return IMRESH_OK;
+}
+
+int vector_reduce(int op, const float *d, uint32_t n, float *out, void *stream)
+{
+    IMR_CHECK(check_n(d, n));
+    if (out == nullptr) return IMRESH_ERR_NULL;
+    hipStream_t s = S(stream);
+    Scratch sc(s);
+    IMR_CHECK(sc.alloc(sizeof(float) * (kReduceBlocks + 64)));
+    float *part = (float *) sc.p, *res = part + kReduceBlocks;
+    IMR_CHECK(launch_reduce(op, d, n, part, kReduceBlocks, res, s));
+    return read_scalar(res, out, s);
+}
+
+int hio_error(const float *g, const void *mask, int u8, uint32_t n, int invert, float *err,
+              float *tot, float *nm, void *stream)
+{
+    if (g == nullptr || mask == nullptr || err == nullptr) return IMRESH_ERR_NULL;
+    hipStream_t s = S(stream);
+    Scratch sc(s);
+    IMR_CHECK(sc.alloc(sizeof(float) * (2 * kReduceBlocks + 64)));
+    float *perr = (float *) sc.p;
+    unsigned *pcnt = (unsigned *) (perr + kReduceBlocks);
+    float *res = perr + 2 * kReduceBlocks;
+    IMR_CHECK(launch_hio_error((const float2 *) g, mask, u8, n, invert, perr, pcnt, kReduceBlocks,
+                               res, s));
+    float h[3];
+    IMR_HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    *err = h[0];
+    if (tot) *tot = h[1];
+    if (nm) *nm = h[2];
+    return IMRESH_OK;
+}
+
+int blur_impl(float *d, uint32_t W, uint32_t H, double sigma, int which, void *stream)
+{
+    if (d == nullptr) return IMRESH_ERR_NULL;
+    if (W == 0 || H == 0) return IMRESH_ERR_INVALID;
+    const int k = gaussian_weights(sigma, nullptr, 0);
+    if (k > 63) return IMRESH_ERR_KERNEL_TOO_LARGE;
+    float w[64 + 1];
+    gaussian_weights(sigma, w, 64);
+    w[64] = 1.f;
+    hipStream_t s = S(stream);
+    Scratch sc(s);
+    const size_t n = (size_t) W * H;
+    IMR_CHECK(sc.alloc(sizeof(float) * (n + 128)));
+    float *dw = (float *) sc.p;
+    float *tmp = dw + 128;
+    IMR_HIP_CHECK(hipMemcpyAsync(dw, w, sizeof(w), hipMemcpyHostToDevice, s));
+    const int r = (k - 1) / 2;
+    if (which == 1 && (size_t) W * 4 + 256 <= 120 * 1024) {
+        IMR_CHECK(launch_blur_h_inplace(d, W, H, dw, r, s));
+    } else {
+        BlurArgs ba{};
+        ba.in = d;
+        ba.out = tmp;
+        ba.wh = (which & 1) ? dw : dw + 64;
+        ba.wv = (which & 2) ? dw : dw + 64;
+        ba.rh = (which & 1) ? r : 0;
+        ba.rv = (which & 2) ? r : 0;
+        ba.h_identity = !(which & 1);
+        ba.v_identity = !(which & 2);
+        ba.W = W;
+        ba.H = H;
+        IMR_CHECK(launch_blur(ba, s));
+        IMR_HIP_CHECK(hipMemcpyAsync(d, tmp, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    }
+    // the host weight array must outlive the copy
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    return IMRESH_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int imresh_abi_version(void) { return IMRESH_ABI_VERSION; }
+
+const char *imresh_strerror(int status)
+{
+    switch (status) {
+    case IMRESH_OK: return "ok";
+    case IMRESH_ERR_NULL: return "null pointer";
+    case IMRESH_ERR_INVALID: return "invalid argument";
+    case IMRESH_ERR_UNSUPPORTED_SIZE: return "unsupported transform size";
+    case IMRESH_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case IMRESH_ERR_NO_DEVICE: return "no HIP device";
+    case IMRESH_ERR_HIP: return "HIP runtime error";
+    case IMRESH_ERR_KERNEL_TOO_LARGE: return "Gaussian kernel wider than 63 taps";
+    case IMRESH_ERR_NOT_INITIALIZED: return "task queue not initialized";
+    case IMRESH_ERR_BUDGET: return "request exceeds the HBM budget";
+    default: return "unknown status";
+    }
+}
+
+int imresh_params_default(imresh_params *p)
+{
+    if (p == nullptr) return IMRESH_ERR_NULL;
+    p->n_cycles = 20;
+    p->target_error = 1e-5f;
+    p->hio_beta = 0.9f;
+    p->cutoff_autocorel = 0.04f;
+    p->cutoff = 0.20f;
+    p->sigma0 = 3.0f;
+    p->sigma_change = 0.01f;
+    p->n_hio_cycles = 20;
+    p->defaults = IMRESH_DEFAULTS_CPU;
+    return IMRESH_OK;
+}
+
+int imresh_device_count(int *count)
+{
+    if (count == nullptr) return IMRESH_ERR_NULL;
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e == hipErrorNoDevice) c = 0;
+    else if (e != hipSuccess) return from_hip(e);
+    *count = c;
+    return IMRESH_OK;
+}
+
+int imresh_dev_alloc(void **dev, uint64_t bytes)
+{
+    if (dev == nullptr) return IMRESH_ERR_NULL;
+    return from_hip(hipMalloc(dev, bytes ? bytes : 1));
+}
+
+int imresh_dev_free(void *dev) { return from_hip(hipFree(dev)); }
+
+int imresh_memcpy_h2d(void *dev, const void *host, uint64_t bytes, void *stream)
+{
+    if (!dev || !host) return IMRESH_ERR_NULL;
+    IMR_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, S(stream)));
+    return from_hip(hipStreamSynchronize(S(stream)));
+}
+
+int imresh_memcpy_d2h(void *host, const void *dev, uint64_t bytes, void *stream)
+{
+    if (!dev || !host) return IMRESH_ERR_NULL;
+    IMR_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, S(stream)));
+    return from_hip(hipStreamSynchronize(S(stream)));
+}
+
+int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream)
+{
+    if (!dst || !src) return IMRESH_ERR_NULL;
+    return from_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)));
+}
+
+int imresh_stream_sync(void *stream) { return from_hip(hipStreamSynchronize(S(stream))); }
+
+int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
+                             const imresh_params *params, void *stream, imresh_result *result)
+{
+    if (io_dev == nullptr) return IMRESH_ERR_NULL;
+    if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
+    Params q;
+    IMR_CHECK(normalize(params, q));
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, width, height, &p));
+    hipStream_t s = S(stream);
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, width, height, s, &w));
+    int st = run_reconstruction(w, io_dev, q, s);
+    if (st == IMRESH_OK && result != nullptr) st = read_result(w, s, result);
+    ws_return(w, s);
+    return st;
+}
+
+int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
+                      const imresh_params *params, void *stream, imresh_result *result)
+{
+    if (io_host == nullptr) return IMRESH_ERR_NULL;
+    if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
+    Params q;
+    IMR_CHECK(normalize(params, q));
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, width, height, &p));
+    hipStream_t s = S(stream);
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, width, height, s, &w));
+    const size_t bytes = sizeof(float) * w->n;
+    int st = from_hip(hipMemcpyAsync(w->M, io_host, bytes, hipMemcpyHostToDevice, s));
+    if (st == IMRESH_OK) st = run_reconstruction(w, w->M, q, s);
+    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(io_host, w->M, bytes, hipMemcpyDeviceToHost, s));
+    if (st == IMRESH_OK) {
+        imresh_result r;
+        st = read_result(w, s, &r);
+        if (st == IMRESH_OK && result) *result = r;
+    }
+    ws_return(w, s);
+    return st;
+}
+
+int imresh_hio_iterations(float *gprime_dev, float *gprev_dev, const uint8_t *mask_dev,
+                          const float *modulus_dev, uint32_t width, uint32_t height,
+                          float hio_beta, uint32_t n_iter, void *stream)
+{
+    if (!gprime_dev || !gprev_dev || !mask_dev || !modulus_dev) return IMRESH_ERR_NULL;
+    if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
+    if (n_iter == 0) return IMRESH_OK;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, width, height, &p));
+    hipStream_t s = S(stream);
+    RowArgs ra{};
+    ra.X = (float2 *) gprime_dev;
+    ra.G = (float2 *) gprev_dev;
+    ra.mask = (uint8_t *) mask_dev;
+    ra.B2 = modulus_dev;   // unused without the cut-off flag; never null in a kernel
+    ra.beta = hio_beta;
+    ColArgs ca{};
+    ca.X = (float2 *) gprime_dev;
+    ca.M = modulus_dev;
+    for (uint32_t it = 0; it < n_iter; ++it) {
+        IMR_CHECK(launch_row(it == 0 ? ROW_FIRST_MASK : ROW_MID, p, ra, s));
+        IMR_CHECK(launch_col(COL_PROJ, p, ca, s));
+    }
+    IMR_CHECK(launch_row(ROW_LAST_PLAIN, p, ra, s));
+    return IMRESH_OK;
+}
+
+int imresh_copy_to_real_part(float *cplx, const float *real, uint32_t n, void *stream)
+{
+    if (!cplx || !real) return IMRESH_ERR_NULL;
+    return launch_copy_to_real((float2 *) cplx, real, n, S(stream));
+}
+
+int imresh_copy_from_real_part(float *real, const float *cplx, uint32_t n, void *stream)
+{
+    if (!cplx || !real) return IMRESH_ERR_NULL;
+    return launch_copy_from_real(real, (const float2 *) cplx, n, S(stream));
+}
+
+int imresh_complex_norm(float *out, const float *cplx, uint32_t n, void *stream)
+{
+    if (!out || !cplx) return IMRESH_ERR_NULL;
+    return launch_complex_norm(out, (const float2 *) cplx, n, S(stream));
+}
+
+int imresh_apply_complex_modulus(float *out, const float *in, const float *modulus, uint32_t n,
+                                 void *stream)
+{
+    if (!out || !in || !modulus) return IMRESH_ERR_NULL;
+    return launch_apply_modulus((float2 *) out, (const float2 *) in, modulus, n, S(stream));
+}
+
+int imresh_hio_domain_constraints(float *gprev, const float *gprime, const float *mask, uint32_t n,
+                                  float hio_beta, void *stream)
+{
+    if (!gprev || !gprime || !mask) return IMRESH_ERR_NULL;
+    return launch_hio_constraint((float2 *) gprev, (const float2 *) gprime, mask, n, hio_beta,
+                                 S(stream));
+}
+
+int imresh_cutoff(float *data, uint32_t n, float threshold, float lower, float upper, void *stream)
+{
+    if (!data) return IMRESH_ERR_NULL;
+    return launch_cutoff(data, n, threshold, lower, upper, S(stream));
+}
+
+int imresh_vector_max(const float *d, uint32_t n, float *r, void *s) { return vector_reduce(0, d, n, r, s); }
+int imresh_vector_min(const float *d, uint32_t n, float *r, void *s) { return vector_reduce(1, d, n, r, s); }
+int imresh_vector_sum(const float *d, uint32_t n, float *r, void *s) { return vector_reduce(2, d, n, r, s); }
+
+int imresh_calculate_hio_error(const float *g, const float *mask, uint32_t n, int invert,
+                               float *err, float *tot, float *nm, void *stream)
+{
+    return hio_error(g, mask, 0, n, invert, err, tot, nm, stream);
+}
+
+int imresh_calculate_hio_error_u8(const float *g, const uint8_t *mask, uint32_t n, int invert,
+                                  float *err, float *tot, float *nm, void *stream)
+{
+    return hio_error(g, mask, 1, n, invert, err, tot, nm, stream);
+}
+
+int imresh_gaussian_kernel(double sigma, float *w, uint32_t capacity)
+{
+    return gaussian_weights(sigma, w, capacity);
+}
+
+int imresh_gaussian_blur(float *d, uint32_t W, uint32_t H, double sigma, void *stream)
+{
+    return blur_impl(d, W, H, sigma, 3, stream);
+}
+
+int imresh_gaussian_blur_horizontal(float *d, uint32_t W, uint32_t H, double sigma, void *stream)
+{
+    return blur_impl(d, W, H, sigma, 1, stream);
+}
+
+int imresh_gaussian_blur_vertical(float *d, uint32_t W, uint32_t H, double sigma, void *stream)
+{
+    return blur_impl(d, W, H, sigma, 2, stream);
+}
+
+int imresh_fft2d(float *out, const float *in, uint32_t W, uint32_t H, int direction, void *stream)
+{
+    if (!out || !in) return IMRESH_ERR_NULL;
+    if (W == 0 || H == 0 || (direction != -1 && direction != 1)) return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, W, H, &p));
+    hipStream_t s = S(stream);
+    RowArgs ra{};
+    ra.in = (const float2 *) in;
+    ra.out = (float2 *) out;
+    IMR_CHECK(launch_row(direction < 0 ? ROW_FWD : ROW_INV, p, ra, s));
+    ColArgs ca{};
+    ca.in = (const float2 *) out;
+    ca.out = (float2 *) out;
+    return launch_col(direction < 0 ? COL_FWD : COL_INV, p, ca, s);
+}
+
+int imresh_fft_supported(uint32_t n) { return line_supported((int) n) ? 1 : 0; }
+
+int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms, void *stream)
+{
+    if (out_ms == nullptr) return IMRESH_ERR_NULL;
+    if (W == 0 || H == 0 || reps == 0) return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, W, H, &p));
+    hipStream_t s = S(stream);
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, W, H, s, &w));
+    int st = IMRESH_OK;
+    Params q;
+    normalize(nullptr, q);
+    q.n_cycles = 1;
+    hipEvent_t ev[5];
+    for (auto &e : ev) hipEventCreate(&e);
+    double acc[4] = {0, 0, 0, 0};
+    do {
+        if ((st = ensure_weights(w, q, s)) != IMRESH_OK) break;
+        if ((st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, W, s)) != IMRESH_OK) break;
+        if ((st = from_hip(hipMemsetAsync(w->flags, 0, 64, s))) != IMRESH_OK) break;
+        RowArgs ra{};
+        ra.X = w->X; ra.G = w->G; ra.mask = w->mask; ra.Bn = w->Bn; ra.B2 = w->B2;
+        ra.thr = w->scal; ra.beta = 0.9f; ra.perr = w->perr; ra.pcnt = w->pcnt;
+        ColArgs ca{};
+        ca.X = w->X; ca.M = w->M;
+        BlurArgs ba{};
+        ba.in = w->Bn; ba.out = w->B2; ba.wh = ba.wv = w->weights;
+        ba.rh = ba.rv = (w->taps[0] - 1) / 2; ba.W = W; ba.H = H; ba.pmax = w->pmax;
+        // one untimed round to warm caches and code objects
+        for (uint32_t r = 0; r <= reps && st == IMRESH_OK; ++r) {
+            hipEventRecord(ev[0], s);
+            st = launch_row(ROW_MID, p, ra, s);
+            hipEventRecord(ev[1], s);
+            if (st == IMRESH_OK) st = launch_col(COL_PROJ, p, ca, s);
+            hipEventRecord(ev[2], s);
+            if (st == IMRESH_OK) st = launch_blur(ba, s);
+            hipEventRecord(ev[3], s);
+            if (st == IMRESH_OK) st = launch_row(ROW_LAST, p, ra, s);
+            hipEventRecord(ev[4], s);
+            if (st != IMRESH_OK) break;
+            if ((st = from_hip(hipEventSynchronize(ev[4]))) != IMRESH_OK) break;
+            if (r == 0) continue;
+            for (int k = 0; k < 4; ++k) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                acc[k] += ms;
+            }
+        }
+    } while (false);
+    for (auto &e : ev) hipEventDestroy(e);
+    hipStreamSynchronize(s);
+    ws_return(w, s);
+    if (st == IMRESH_OK)
+        for (int k = 0; k < 4; ++k) out_ms[k] = (float) (acc[k] / reps);
+    return st;
+}
+
+}  // extern "C"

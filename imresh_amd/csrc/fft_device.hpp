@@ -1,0 +1,326 @@
+// Device building blocks of the LDS-staged Stockham line FFT for gfx950.
+//
+// A line of length n = prod(R_p) is transformed by T threads (compile-time
+// plan, see plans.hpp).  Pass p (radix R, span ns = R_0*...*R_{p-1}) owns
+// butterflies j = t + b*T; it reads elements j + r*(n/R), multiplies element r
+// by exp(-+2*pi*i*r*(j%ns)/(ns*R)), runs a radix-R DFT in registers and
+// writes element r to (j - j%ns)*R + j%ns + r*ns.  The first pass therefore
+// reads and the last pass writes the line in natural order j + r*(n/R): the
+// accesses are coalesced in global memory, and a fused prologue/epilogue
+// (HIO constraint, modulus projection, norm, error sums) runs on registers at
+// exactly those positions.  Between passes the data goes through LDS; each
+// thread keeps at most E = 16 complex values in VGPRs.
+//
+// Forward = exp(-2*pi*i*j*k/n) (FFTW_FORWARD / CUFFT_FORWARD); both
+// directions are unnormalized, as the reference's FFTs.  Forward transforms
+// run the radices in plan order, inverse transforms in reverse order, so the
+// last pass of one direction and the first pass of the other have the same
+// radix and butterfly ownership: a forward->inverse (or inverse->forward)
+// pair with an elementwise operator in between never leaves registers at the
+// junction (line_pair).
+#pragma once
+
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace imr {
+
+// native 2-wide vector: one 64-bit load / store per complex value
+typedef float cf __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cf mkc(float x, float y)
+{
+    cf r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+
+// ------------------------------------------------------------ complex math
+// Products use one fma per component; the FFT is the only place that
+// contracts -- the library is compiled with -ffp-contract=off so that all
+// elementwise operators round exactly as the reference's C++ code.
+__device__ __forceinline__ cf cadd(cf a, cf b) { return mkc(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cf csub(cf a, cf b) { return mkc(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cf cmul(cf a, cf b)
+{
+    return mkc(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
+}
+// a * conj(b)
+__device__ __forceinline__ cf cmulc(cf a, cf b)
+{
+    return mkc(__builtin_fmaf(a.x, b.x, a.y * b.y), __builtin_fmaf(a.y, b.x, -(a.x * b.y)));
+}
+// a * (DIR < 0 ? -i : +i)
+template <int DIR> __device__ __forceinline__ cf rot(cf a)
+{
+    return DIR < 0 ? mkc(a.y, -a.x) : mkc(-a.y, a.x);
+}
+// twiddle multiply: forward uses the table value, inverse its conjugate
+template <int DIR> __device__ __forceinline__ cf twmul(cf a, cf w)
+{
+    return DIR < 0 ? cmul(a, w) : cmulc(a, w);
+}
+
+// cos/sin(2*pi*k/R) for the odd radices
+template <int R> struct OddConst;
+template <> struct OddConst<3> {
+    static constexpr float c[3] = {1.f, -0.5f, -0.5f};
+    static constexpr float s[3] = {0.f, 0.866025404f, -0.866025404f};
+};
+template <> struct OddConst<5> {
+    static constexpr float c[5] = {1.f, 0.309016994f, -0.809016994f, -0.809016994f, 0.309016994f};
+    static constexpr float s[5] = {0.f, 0.951056516f, 0.587785252f, -0.587785252f, -0.951056516f};
+};
+
+// ------------------------------------------------------------ butterflies
+template <int DIR>
+__device__ __forceinline__ void bf2(cf *v)
+{
+    const cf a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void bf4io(cf a, cf b, cf c, cf d, cf &o0, cf &o1, cf &o2, cf &o3)
+{
+    const cf t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = rot<DIR>(csub(b, d));
+    o0 = cadd(t0, t2);
+    o2 = csub(t0, t2);
+    o1 = cadd(t1, t3);
+    o3 = csub(t1, t3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void bf4(cf *v)
+{
+    cf o0, o1, o2, o3;
+    bf4io<DIR>(v[0], v[1], v[2], v[3], o0, o1, o2, o3);
+    v[0] = o0; v[1] = o1; v[2] = o2; v[3] = o3;
+}
+
+template <int DIR>
+__device__ __forceinline__ void bf8(cf *v)
+{
+    constexpr float c = 0.707106781f;
+    cf e0, e1, e2, e3, o0, o1, o2, o3;
+    bf4io<DIR>(v[0], v[2], v[4], v[6], e0, e1, e2, e3);
+    bf4io<DIR>(v[1], v[3], v[5], v[7], o0, o1, o2, o3);
+    o1 = cmul(o1, mkc(c, DIR * c));
+    o2 = rot<DIR>(o2);
+    o3 = cmul(o3, mkc(-c, DIR * c));
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void bf16(cf *v)
+{
+    constexpr float c1 = 0.923879533f, s1 = 0.382683432f, c2 = 0.707106781f;
+    cf y[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        bf4io<DIR>(v[q], v[q + 4], v[q + 8], v[q + 12], y[4 * q + 0], y[4 * q + 1], y[4 * q + 2],
+                   y[4 * q + 3]);
+    // y[4*q + s] *= W16^(q*s), W16^k = (cos(2 pi k/16), DIR sin(2 pi k/16))
+    y[5] = cmul(y[5], mkc(c1, DIR * s1));     // k = 1
+    y[6] = cmul(y[6], mkc(c2, DIR * c2));     // k = 2
+    y[7] = cmul(y[7], mkc(s1, DIR * c1));     // k = 3
+    y[9] = cmul(y[9], mkc(c2, DIR * c2));     // k = 2
+    y[10] = rot<DIR>(y[10]);                  // k = 4
+    y[11] = cmul(y[11], mkc(-c2, DIR * c2));  // k = 6
+    y[13] = cmul(y[13], mkc(s1, DIR * c1));   // k = 3
+    y[14] = cmul(y[14], mkc(-c2, DIR * c2));  // k = 6
+    y[15] = cmul(y[15], mkc(-c1, -DIR * s1)); // k = 9
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        bf4io<DIR>(y[s], y[4 + s], y[8 + s], y[12 + s], v[s], v[s + 4], v[s + 8], v[s + 12]);
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void bf_odd(cf *v)
+{
+    constexpr int H = (R - 1) / 2;
+    cf t[H], u[H];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        t[k - 1] = cadd(v[k], v[R - k]);
+        u[k - 1] = csub(v[k], v[R - k]);
+    }
+    const cf x0 = v[0];
+    cf sum = x0;
+#pragma unroll
+    for (int k = 0; k < H; ++k) sum = cadd(sum, t[k]);
+#pragma unroll
+    for (int s = 1; s <= H; ++s) {
+        cf a = x0, b = mkc(0.f, 0.f);
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            const float cc = OddConst<R>::c[(k * s) % R];
+            const float ss = OddConst<R>::s[(k * s) % R];
+            a = mkc(__builtin_fmaf(t[k - 1].x, cc, a.x), __builtin_fmaf(t[k - 1].y, cc, a.y));
+            b = mkc(__builtin_fmaf(u[k - 1].x, ss, b.x), __builtin_fmaf(u[k - 1].y, ss, b.y));
+        }
+        const cf rb = rot<DIR>(b);
+        v[s] = cadd(a, rb);
+        v[R - s] = csub(a, rb);
+    }
+    v[0] = sum;
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void bfly(cf *v)
+{
+    if constexpr (R == 1) (void) v;   // length-1 line: identity
+    else if constexpr (R == 2) bf2<DIR>(v);
+    else if constexpr (R == 4) bf4<DIR>(v);
+    else if constexpr (R == 8) bf8<DIR>(v);
+    else if constexpr (R == 16) bf16<DIR>(v);
+    else bf_odd<R, DIR>(v);
+}
+
+// ------------------------------------------------------------ LDS layout
+// One padding slot every 32 complex values spreads the stride-R scatters of
+// the early passes over the LDS banks.
+__host__ __device__ constexpr int lpad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lds_elems(int n) { return n + (n >> 5) + 1; }
+
+struct LdsLine {
+    cf *p;
+    int base;
+    int stride;
+    __device__ __forceinline__ cf &operator()(int i) const { return p[lpad(base + i * stride)]; }
+};
+
+// ------------------------------------------------------------ passes
+// Pass I of plan P in direction DIR (sequence order: forward = plan order,
+// inverse = reversed).  SRC: gather from `src` (global) instead of LDS;
+// DST: scatter to `dst` (global) instead of LDS.
+template <class P, int DIR, int I, bool SRC, bool DST, class Src, class Dst>
+__device__ __forceinline__ void cpass(cf (&v)[P::E], const cf *__restrict__ tw, int t,
+                                      const LdsLine &L, Src &src, Dst &dst)
+{
+    constexpr bool INV = DIR > 0;
+    constexpr int R = P::rad(INV, I), NS = P::ns(INV, I), TWOFF = P::twoff(INV, I);
+    constexpr int T = P::T, nb = P::n / R, B = (nb + T - 1) / T;
+    static_assert(B * R <= P::E, "plan exceeds the per-thread element budget");
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = t + b * T;
+        if (nb % T == 0 || j < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (SRC) v[b * R + r] = src(j + r * nb);
+                else v[b * R + r] = L(j + r * nb);
+            }
+        }
+    }
+    if constexpr (!SRC) __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = t + b * T;
+        if (nb % T == 0 || j < nb) {
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+                const cf *w = tw + TWOFF + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b * R + r] = twmul<DIR>(v[b * R + r], w[r - 1]);
+            }
+            bfly<R, DIR>(&v[b * R]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (DST) dst(base + r * NS, v[b * R + r]);
+                else L(base + r * NS) = v[b * R + r];
+            }
+        }
+    }
+    if constexpr (!DST) __syncthreads();
+}
+
+// passes [I, I1) of direction DIR; pass 0 gathers from src, pass np-1
+// scatters to dst, everything else goes through LDS
+template <class P, int DIR, int I, int I1, class Src, class Dst>
+__device__ __forceinline__ void cpasses(cf (&v)[P::E], const cf *tw, int t, const LdsLine &L,
+                                        Src &src, Dst &dst)
+{
+    if constexpr (I < I1) {
+        cpass<P, DIR, I, I == 0, I == P::np - 1>(v, tw, t, L, src, dst);
+        cpasses<P, DIR, I + 1, I1>(v, tw, t, L, src, dst);
+    }
+}
+
+// One full transform: global (src) -> ... -> global (dst).
+template <class P, int DIR, class Src, class Dst>
+__device__ __forceinline__ void line_fft(cf (&v)[P::E], const cf *tw, int t, const LdsLine &L,
+                                         Src &src, Dst &dst)
+{
+    cpasses<P, DIR, 0, P::np>(v, tw, t, L, src, dst);
+}
+
+// Transform in direction DA, elementwise `mid` on its natural-order output,
+// then the transform in direction DB -- the junction stays in registers.
+template <class P, int DA, int DB, class Src, class Mid, class Dst>
+__device__ __forceinline__ void line_pair(cf (&v)[P::E], const cf *tw, int t, const LdsLine &L,
+                                          Src &src, Mid &mid, Dst &dst)
+{
+    static_assert(DA == -DB, "a pair runs opposite directions");
+    constexpr int np = P::np, T = P::T;
+    constexpr bool INVA = DA > 0;
+    // all passes of A but the last
+    cpasses<P, DA, 0, np - 1>(v, tw, t, L, src, dst);
+    // last pass of A (radix R) + mid + first pass of B (same radix, span 1)
+    constexpr int I = np - 1;
+    constexpr int R = P::rad(INVA, I), NS = P::ns(INVA, I), TWOFF = P::twoff(INVA, I);
+    static_assert(R == P::rad(!INVA, 0), "mirrored sequences share the junction radix");
+    constexpr int nb = P::n / R, B = (nb + T - 1) / T;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = t + b * T;
+        if (nb % T == 0 || j < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (np == 1) v[b * R + r] = src(j + r * nb);
+                else v[b * R + r] = L(j + r * nb);
+            }
+        }
+    }
+    if constexpr (np > 1) __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = t + b * T;
+        if (nb % T == 0 || j < nb) {
+            if constexpr (NS > 1) {
+                const int k = j % NS;
+                const cf *w = tw + TWOFF + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b * R + r] = twmul<DA>(v[b * R + r], w[r - 1]);
+            }
+            bfly<R, DA>(&v[b * R]);
+            // the last pass writes natural order j + r*nb (j < nb == NS)
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = mid(j + r * nb, v[b * R + r]);
+            bfly<R, DB>(&v[b * R]);
+            // first pass of B: span 1, output j*R + r
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (np == 1) dst(j * R + r, v[b * R + r]);
+                else L(j * R + r) = v[b * R + r];
+            }
+        }
+    }
+    if constexpr (np > 1) {
+        __syncthreads();
+        cpasses<P, DB, 1, np>(v, tw, t, L, src, dst);
+    }
+}
+
+// ------------------------------------------------------------ reductions
+// std::max / std::min semantics of the reference reductions
+__device__ __forceinline__ float fmax_ref(float m, float x) { return (m < x) ? x : m; }
+__device__ __forceinline__ float fmin_ref(float m, float x) { return (x < m) ? x : m; }
+
+}  // namespace imr

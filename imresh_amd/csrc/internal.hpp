@@ -1,0 +1,138 @@
+// Internal interfaces between the kernel translation unit, the planner and
+// the engine (not part of the C ABI).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace imr {
+
+// ---------------------------------------------------------------- kernels
+enum RowMode {
+    ROW_FWD = 0,        // in -> out, forward row transform
+    ROW_INV,            // in -> out, inverse row transform
+    ROW_INIT,           // X -> inverse -> X = g', G = g', Bn = |g'|
+    ROW_FIRST_CUT,      // X = g'; mask = B2 < thr; G = constraint; forward -> X
+    ROW_FIRST_MASK,     // X = g'; G = constraint(mask); forward -> X
+    ROW_MID,            // X -> inverse -> constraint(mask, G) -> forward -> X
+    ROW_LAST,           // X -> inverse -> X = g', Bn = |g'|, error partials
+    ROW_LAST_PLAIN,     // X -> inverse -> X = g'
+    ROW_NMODES
+};
+
+enum ColMode {
+    COL_FWD = 0,        // in -> out, forward column transform
+    COL_INV,            // in -> out, inverse column transform
+    COL_PROJ,           // X -> forward -> modulus projection -> inverse -> X
+    COL_INIT,           // real M -> inverse -> X
+    COL_NMODES
+};
+
+struct RowArgs {
+    const float2 *in;
+    float2 *out;
+    float2 *X;
+    float2 *G;
+    uint8_t *mask;
+    float *Bn;
+    const float *B2;
+    const float *thr;      // device scalar: mask threshold
+    float beta;
+    float *perr;           // per-workgroup sum mask*|g'|
+    unsigned *pcnt;        // per-workgroup sum mask
+    const int *done;       // early-exit flag (NULL: never)
+    int flags;             // kRow* bits (set by launch_row from the mode)
+    int H;                 // number of rows
+    const float2 *tw;      // row-length twiddle table (set by launch_row)
+};
+
+struct ColArgs {
+    const float2 *in;
+    float2 *out;
+    float2 *X;
+    const float *M;        // modulus (COL_PROJ) or real input (COL_INIT)
+    const int *done;
+    int flags;             // kCol* bits (set by launch_col from the mode)
+    int W;                 // number of columns = row pitch
+    const float2 *tw;      // column-length twiddle table (set by launch_col)
+};
+
+struct BlurArgs {
+    const float *in;
+    float *out;
+    const float *wh;       // horizontal weights (2*rh+1)
+    const float *wv;       // vertical weights (2*rv+1)
+    int rh, rv;
+    int h_identity, v_identity;
+    int W, H;
+    float *pmax;           // per-workgroup max (NULL: none)
+    const int *done;
+};
+
+// A compile-time plan (plans.hpp) of one line length with its twiddles.
+struct LinePlan {
+    int n;
+    int group;       // translation-unit group of the kernel instantiations
+    int T;           // threads per line
+    int L;           // rows per row workgroup
+    int C;           // columns per column workgroup
+    const float2 *tw;
+};
+
+struct Plan2D {
+    int W, H;
+    LinePlan row;    // n = W
+    LinePlan col;    // n = H
+};
+
+int get_line_plan(int device, int n, LinePlan *out);
+int get_plan2d(int device, int W, int H, Plan2D *out);
+bool line_supported(int n);
+
+int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s);
+int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s);
+// per translation-unit group (k_row.hip / k_col.hip compiled once per group)
+#define IMR_DECLARE_GROUP(g)                                                         \
+    int launch_row_g##g(int mode, int n, const LinePlan &lp, RowArgs a, hipStream_t s); \
+    int launch_col_g##g(int mode, int n, const LinePlan &lp, ColArgs a, hipStream_t s);
+IMR_DECLARE_GROUP(0)
+IMR_DECLARE_GROUP(1)
+IMR_DECLARE_GROUP(2)
+IMR_DECLARE_GROUP(3)
+IMR_DECLARE_GROUP(4)
+int row_grid(const Plan2D &p);
+int blur_grid(int W, int H);
+int launch_blur(const BlurArgs &a, hipStream_t s);
+int launch_blur_h_inplace(float *d, int W, int H, const float *w, int r, hipStream_t s);
+int launch_max_finalize(const float *pmax, int np, float cut, float *thr, const int *done,
+                        hipStream_t s);
+int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float target,
+                        float *res, int *done, unsigned *cycles, hipStream_t s);
+int launch_copy_from_real(float *out, const float2 *in, size_t n, hipStream_t s);
+int launch_copy_to_real(float2 *out, const float *in, size_t n, hipStream_t s);
+int launch_complex_norm(float *out, const float2 *in, size_t n, hipStream_t s);
+int launch_apply_modulus(float2 *out, const float2 *in, const float *M, size_t n, hipStream_t s);
+int launch_hio_constraint(float2 *gp, const float2 *g, const float *mask, size_t n, float beta,
+                          hipStream_t s);
+int launch_cutoff(float *d, size_t n, float t, float lo, float hi, hipStream_t s);
+// op: 0 = max, 1 = min, 2 = sum.  result: device float.
+int launch_reduce(int op, const float *d, size_t n, float *partials, int np, float *result,
+                  hipStream_t s);
+int launch_hio_error(const float2 *g, const void *mask, int mask_is_u8, size_t n, int invert,
+                     float *perr, unsigned *pcnt, int np, float *res, hipStream_t s);
+int launch_fill_state(float2 *X, float2 *G, uint8_t *mask, float *M, float *Bn, size_t n,
+                      int W, hipStream_t s);
+
+constexpr int kReduceBlocks = 512;
+constexpr int kBlurTX = 64;
+constexpr int kBlurTY = 32;
+
+// host Gaussian weights: calcGaussianKernel semantics (calcGaussianKernel.cpp:42-91)
+int gaussian_weights(double sigma, float *w, unsigned capacity);
+// sigma_{c+1} = max(1.5, (1 - dsigma) * sigma_c) in float (shrinkWrap.cpp:245)
+void sigma_schedule(float sigma0, float dsigma, unsigned n, std::vector<float> &out);
+
+}  // namespace imr

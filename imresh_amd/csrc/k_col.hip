@@ -1,0 +1,126 @@
+// Column-pass kernels (one workgroup = P::C adjacent columns of length
+// H = P::n, the tile staged in LDS as [H][C], P::T threads per column).
+// Compiled once per plan group (-DIMR_GROUP=g).
+//
+//   CK_PROJ  forward column FFT -> modulus projection -> inverse column FFT.
+//            Finishes the forward 2-D FFT of an HIO iteration, replaces |G'|
+//            by the measured modulus (cudaVectorElementwise.cu:168-195) in
+//            registers, and starts the inverse 2-D FFT.
+//   CK_INV   inverse column transform; with kColRealIn the input is the real
+//            modulus M, i.e. the first half of IFFT2((M, 0))
+//            (cudaShrinkWrap.cu:229-232).
+//   CK_FWD   forward column transform of the 2-D FFT entry point.
+#include "internal.hpp"
+#include "ops_device.hpp"
+#include "plans.hpp"
+
+#ifndef IMR_GROUP
+#error "compile with -DIMR_GROUP=<plan group>"
+#endif
+
+namespace imr {
+
+namespace {
+
+enum ColKernel { CK_FWD, CK_INV, CK_PROJ };
+
+template <class P, int K>
+__global__ void __launch_bounds__(P::C *P::T) k_col(ColArgs a)
+{
+    if (a.done != nullptr && *a.done) return;
+    extern __shared__ cf smem[];
+    constexpr int C = P::C;
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const int col = blockIdx.x * C + c;
+    const bool valid = col < a.W;
+    const int W = a.W;
+    // workgroup base (uniform) + 32-bit per-lane offsets e*W + c; columns past
+    // the edge read column 0 of the tile and never store
+    const size_t wb = (size_t) blockIdx.x * C;
+    const int co = valid ? c : 0;
+    const LdsLine L{smem, c, C};
+    const cf *tw = (const cf *) a.tw;
+    cf v[P::E];
+
+    if constexpr (K == CK_FWD) {
+        const cf *in = (const cf *) a.in + wb;
+        cf *out = (cf *) a.out + wb;
+        auto g0 = [=](int e) { return in[e * W + co]; };
+        auto sl = [=](int e, cf x) { if (valid) out[e * W + co] = x; };
+        line_fft<P, -1>(v, tw, t, L, g0, sl);
+    } else if constexpr (K == CK_INV) {
+        const bool real_in = (a.flags & kColRealIn) != 0;
+        const cf *in = (const cf *) a.in + wb;
+        const float *M = a.M + wb;
+        cf *out = (cf *) a.out + wb;
+        auto g0 = [=](int e) { return real_in ? mkc(M[e * W + co], 0.f) : in[e * W + co]; };
+        auto sl = [=](int e, cf x) { if (valid) out[e * W + co] = x; };
+        line_fft<P, +1>(v, tw, t, L, g0, sl);
+    } else if constexpr (K == CK_PROJ) {
+        cf *X = (cf *) a.X + wb;
+        const float *M = a.M + wb;
+        auto g0 = [=](int e) { return X[e * W + co]; };
+        auto mid = [=](int e, cf z) { return modulus_project(z, M[e * W + co]); };
+        auto sl = [=](int e, cf x) { if (valid) X[e * W + co] = x; };
+        line_pair<P, -1, +1>(v, tw, t, L, g0, mid, sl);
+    }
+}
+
+template <class P>
+int launch_plan(int mode, ColArgs a, hipStream_t s)
+{
+    const dim3 grid((a.W + P::C - 1) / P::C), block(P::C * P::T);
+    const size_t lds = sizeof(cf) * (size_t) lds_elems(P::C * P::n);
+    if (lds > 64 * 1024) {
+        static const hipError_t attr = [&] {
+            hipError_t e = hipSuccess;
+            for (const void *f : {(const void *) k_col<P, CK_FWD>, (const void *) k_col<P, CK_INV>,
+                                  (const void *) k_col<P, CK_PROJ>})
+                if (e == hipSuccess)
+                    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+            return e;
+        }();
+        if (attr != hipSuccess) return from_hip(attr);
+    }
+    switch (mode) {
+    case COL_FWD: a.flags = 0; hipLaunchKernelGGL((k_col<P, CK_FWD>), grid, block, lds, s, a); break;
+    case COL_INV: a.flags = 0; hipLaunchKernelGGL((k_col<P, CK_INV>), grid, block, lds, s, a); break;
+    case COL_INIT:
+        a.flags = kColRealIn;
+        a.in = a.X;   // unused with kColRealIn; never a null pointer in a kernel
+        a.out = a.X;
+        hipLaunchKernelGGL((k_col<P, CK_INV>), grid, block, lds, s, a);
+        break;
+    case COL_PROJ: a.flags = 0; hipLaunchKernelGGL((k_col<P, CK_PROJ>), grid, block, lds, s, a); break;
+    default: return IMRESH_ERR_INVALID;
+    }
+    return from_hip(hipGetLastError());
+}
+
+template <int G, int N, int... Rs>
+int try_plan(int mode, int n, ColArgs a, hipStream_t s)
+{
+    if constexpr (G == IMR_GROUP) {
+        if (n == N) return launch_plan<Plan<N, Rs...>>(mode, a, s);
+    }
+    return -1;
+}
+
+}  // namespace
+
+#define IMR_CAT2(a, b) a##b
+#define IMR_CAT(a, b) IMR_CAT2(a, b)
+#define IMR_COL_CASE(g, N, ...)                                                     \
+    {                                                                               \
+        const int r = try_plan<g, N, __VA_ARGS__>(mode, n, a, s);                   \
+        if (r >= 0) return r;                                                       \
+    }
+
+int IMR_CAT(launch_col_g, IMR_GROUP)(int mode, int n, const LinePlan &lp, ColArgs a, hipStream_t s)
+{
+    a.tw = lp.tw;
+    IMR_PLANS(IMR_COL_CASE)
+    return IMRESH_ERR_UNSUPPORTED_SIZE;
+}
+
+}  // namespace imr

@@ -1,0 +1,60 @@
+// Elementwise device operators of the hot path and a deterministic
+// workgroup reduction.  The library is compiled with -ffp-contract=off and
+// IEEE sqrtf / division, so each operator rounds exactly like the
+// reference's C++ code given identical inputs:
+//   constraint   cudaVectorElementwise.cu:42-72, shrinkWrap.cpp:263-276
+//   projection   cudaVectorElementwise.cu:168-195, vectorElementwise.cpp:59-81
+//   norm         cudaVectorElementwise.cu:120-141, vectorElementwise.cpp:41-56
+#pragma once
+
+#include "fft_device.hpp"
+
+namespace imr {
+
+// if (masked or Re g' < 0): gPrev -= beta*g'  else gPrev = g'
+__device__ __forceinline__ cf hio_constraint(cf gp, cf g, bool masked, float beta)
+{
+    if (masked || g.x < 0) {
+        gp.x -= beta * g.x;
+        gp.y -= beta * g.y;
+        return gp;
+    }
+    return g;
+}
+
+__device__ __forceinline__ float cnorm(cf z) { return sqrtf(z.x * z.x + z.y * z.y); }
+
+// |z| := m, keeping the phase; z == 0 stays 0
+__device__ __forceinline__ cf modulus_project(cf z, float m)
+{
+    float norm = sqrtf(z.x * z.x + z.y * z.y);
+    if (norm == 0) norm = 1;
+    const float f = m / norm;
+    return mkc(z.x * f, z.y * f);
+}
+
+// deterministic workgroup reduction through LDS (any blockDim <= 1024):
+// a fixed pairwise tree, so results do not depend on scheduling
+template <class T, class Op>
+__device__ __forceinline__ T block_reduce(T x, T *red, Op op)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    __syncthreads();
+    red[tid] = x;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (tid < s && tid + s < nt) red[tid] = op(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// RowArgs.flags
+constexpr int kRowWriteG = 1;     // write gPrev = g'
+constexpr int kRowWriteBn = 2;    // write |g'| (blur input)
+constexpr int kRowError = 4;      // per-workgroup HIO error sums
+constexpr int kRowCut = 8;        // mask = blurred < threshold (else read mask)
+// ColArgs.flags
+constexpr int kColRealIn = 1;     // input is the real modulus M
+
+}  // namespace imr

@@ -1,0 +1,176 @@
+// Host-side planning: the compile-time plan registry (plans.hpp) viewed from
+// the host, twiddle tables per device and length (cached for the process),
+// kernel dispatch by plan group, and the Gaussian weights / sigma schedule
+// of the reference (computed on the host exactly as the reference does, so
+// the device blur uses bit-identical weights).
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "internal.hpp"
+#include "plans.hpp"
+
+namespace imr {
+
+namespace {
+
+std::mutex g_plan_mtx;
+std::map<std::pair<int, int>, LinePlan> g_lines;   // (device, n) -> plan (twiddles on device)
+
+template <int G, int N, int... Rs>
+PlanInfo make_info()
+{
+    using P = Plan<N, Rs...>;
+    PlanInfo pi{};
+    pi.group = G;
+    pi.n = N;
+    pi.np = P::np;
+    pi.T = P::T;
+    pi.L = P::L;
+    pi.C = P::C;
+    for (int i = 0; i < P::np; ++i) pi.rad[i] = P::rf[i];
+    return pi;
+}
+
+#define IMR_INFO_ENTRY(g, N, ...) make_info<g, N, __VA_ARGS__>(),
+
+const std::vector<PlanInfo> &registry()
+{
+    static const std::vector<PlanInfo> r = {IMR_PLANS(IMR_INFO_ENTRY)};
+    return r;
+}
+
+const PlanInfo *find_info(int n)
+{
+    for (const auto &pi : registry())
+        if (pi.n == n) return &pi;
+    return nullptr;
+}
+
+// exp(-2 pi i r k / (ns R)) for every pass of a radix sequence, evaluated in
+// double; layout as Plan::twoff (k-major, r = 1..R-1)
+void append_twiddles(const std::vector<int> &rad, std::vector<float2> &tw)
+{
+    int ns = 1;
+    for (int R : rad) {
+        for (int k = 0; k < ns; ++k)
+            for (int r = 1; r < R; ++r) {
+                const double a = -2.0 * M_PI * (double) r * (double) k / ((double) ns * R);
+                tw.push_back(make_float2((float) std::cos(a), (float) std::sin(a)));
+            }
+        ns *= R;
+    }
+}
+
+}  // namespace
+
+bool line_supported(int n) { return find_info(n) != nullptr; }
+
+int get_line_plan(int device, int n, LinePlan *out)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    auto it = g_lines.find({device, n});
+    if (it != g_lines.end()) {
+        *out = it->second;
+        return IMRESH_OK;
+    }
+    const PlanInfo *pi = find_info(n);
+    if (pi == nullptr) return IMRESH_ERR_UNSUPPORTED_SIZE;
+    std::vector<int> fwd(pi->rad, pi->rad + pi->np), inv(fwd.rbegin(), fwd.rend());
+    std::vector<float2> tw;
+    append_twiddles(fwd, tw);
+    append_twiddles(inv, tw);
+    tw.push_back(make_float2(0.f, 0.f));   // never empty
+    float2 *dev = nullptr;
+    int prev = 0;
+    hipGetDevice(&prev);
+    hipSetDevice(device);
+    hipError_t e = hipMalloc(&dev, sizeof(float2) * tw.size());
+    if (e == hipSuccess) e = hipMemcpy(dev, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
+    hipSetDevice(prev);
+    if (e != hipSuccess) return from_hip(e);
+    LinePlan lp{};
+    lp.n = n;
+    lp.group = pi->group;
+    lp.T = pi->T;
+    lp.L = pi->L;
+    lp.C = pi->C;
+    lp.tw = dev;
+    g_lines[{device, n}] = lp;
+    *out = lp;
+    return IMRESH_OK;
+}
+
+int get_plan2d(int device, int W, int H, Plan2D *out)
+{
+    if (W < 1 || H < 1) return IMRESH_ERR_INVALID;
+    Plan2D p{};
+    p.W = W;
+    p.H = H;
+    IMR_CHECK(get_line_plan(device, W, &p.row));
+    IMR_CHECK(get_line_plan(device, H, &p.col));
+    *out = p;
+    return IMRESH_OK;
+}
+
+int row_grid(const Plan2D &p) { return (p.H + p.row.L - 1) / p.row.L; }
+
+int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
+{
+    a.H = p.H;
+    switch (p.row.group) {
+    case 0: return launch_row_g0(mode, p.W, p.row, a, s);
+    case 1: return launch_row_g1(mode, p.W, p.row, a, s);
+    case 2: return launch_row_g2(mode, p.W, p.row, a, s);
+    case 3: return launch_row_g3(mode, p.W, p.row, a, s);
+    case 4: return launch_row_g4(mode, p.W, p.row, a, s);
+    default: return IMRESH_ERR_UNSUPPORTED_SIZE;
+    }
+}
+
+int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
+{
+    a.W = p.W;
+    switch (p.col.group) {
+    case 0: return launch_col_g0(mode, p.H, p.col, a, s);
+    case 1: return launch_col_g1(mode, p.H, p.col, a, s);
+    case 2: return launch_col_g2(mode, p.H, p.col, a, s);
+    case 3: return launch_col_g3(mode, p.H, p.col, a, s);
+    case 4: return launch_col_g4(mode, p.H, p.col, a, s);
+    default: return IMRESH_ERR_UNSUPPORTED_SIZE;
+    }
+}
+
+// calcGaussianKernel.cpp:42-91: radius ceil(2.884402748387961466*sigma - 0.5),
+// float weights a*exp(i^2*b) summed in double, each divided by the sum.
+int gaussian_weights(double sigma, float *w, unsigned capacity)
+{
+    const int nNeighbors = (int) std::ceil(2.884402748387961466 * sigma - 0.5);
+    const int nWeights = 2 * nNeighbors + 1;
+    if (w == nullptr || (unsigned) nWeights > capacity) return nWeights;
+    double sum = 0;
+    const double a = 1.0 / (std::sqrt(2.0 * M_PI) * sigma);
+    const double b = -1.0 / (2.0 * sigma * sigma);
+    for (int i = -nNeighbors; i <= nNeighbors; ++i) {
+        const float weight = (float) (a * std::exp(i * i * b));
+        w[nNeighbors + i] = weight;
+        sum += weight;
+    }
+    for (int i = -nNeighbors; i <= nNeighbors; ++i)
+        w[nNeighbors + i] = (float) ((double) w[nNeighbors + i] / sum);
+    return nWeights;
+}
+
+void sigma_schedule(float sigma0, float dsigma, unsigned n, std::vector<float> &out)
+{
+    out.resize(n);
+    float sigma = sigma0;
+    for (unsigned i = 0; i < n; ++i) {
+        out[i] = sigma;
+        sigma = std::fmax(1.5f, (1.0f - dsigma) * sigma);
+    }
+}
+
+}  // namespace imr

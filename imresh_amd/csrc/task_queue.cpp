@@ -1,0 +1,217 @@
+// Multi-stream task queue and the C++ drop-in API.
+//
+// Reference behaviour (src/imresh/io/taskQueue.cu:91-251): taskQueueInit
+// creates 3 streams per device; addTask starts one std::thread per task and,
+// when as many threads as streams exist, joins the OLDEST one first; each
+// thread takes the least recently used stream, runs cudaShrinkWrap (with
+// per-call cudaMalloc / cufftPlan2d) and then the write-out callback.
+//
+// Here: one persistent worker thread per (device, stream) slot, each with a
+// reusable device workspace (the engine's workspace pool), fed from a
+// bounded FIFO.  addTask blocks while the FIFO holds as many pending tasks as
+// there are slots, so at most 2 x slots tasks are in flight (the reference
+// allowed `slots`); callbacks run on the worker threads, as in the reference.
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "imresh/algorithms/shrinkWrap.hpp"
+#include "imresh/io/taskQueue.hpp"
+#include "internal.hpp"
+
+namespace {
+
+struct Task {
+    float *io;
+    uint32_t w, h;
+    imresh_params params;
+    std::function<void(float *, uint32_t, uint32_t, int)> done;
+};
+
+struct Slot {
+    int device;
+    hipStream_t stream;
+    std::thread worker;
+};
+
+struct Queue {
+    std::mutex mtx;
+    std::condition_variable cv_work, cv_space;
+    std::deque<Task> pending;
+    std::vector<Slot> slots;
+    size_t capacity = 0;
+    bool running = false;
+    bool closing = false;
+} g_q;
+
+std::mutex g_init_mtx;
+
+void worker_main(size_t idx)
+{
+    const Slot &slot = g_q.slots[idx];
+    hipSetDevice(slot.device);
+    for (;;) {
+        Task t;
+        {
+            std::unique_lock<std::mutex> lk(g_q.mtx);
+            g_q.cv_work.wait(lk, [] { return g_q.closing || !g_q.pending.empty(); });
+            if (g_q.pending.empty()) return;   // closing and drained
+            t = std::move(g_q.pending.front());
+            g_q.pending.pop_front();
+        }
+        g_q.cv_space.notify_one();
+        const int st = imresh_shrinkwrap(t.io, t.w, t.h, &t.params, (void *) slot.stream, nullptr);
+        if (t.done) t.done(t.io, t.w, t.h, st);
+    }
+}
+
+int queue_init(int per_device)
+{
+    std::lock_guard<std::mutex> lk(g_init_mtx);
+    if (g_q.running) return IMRESH_OK;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return IMRESH_ERR_NO_DEVICE;
+    if (per_device <= 0) per_device = 3;   // taskQueue.cu:217
+    g_q.closing = false;
+    g_q.slots.clear();
+    for (int d = 0; d < ndev; ++d) {
+        hipSetDevice(d);
+        for (int j = 0; j < per_device; ++j) {
+            Slot s;
+            s.device = d;
+            e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+            if (e != hipSuccess) return imr::from_hip(e);
+            g_q.slots.push_back(std::move(s));
+        }
+    }
+    hipSetDevice(0);
+    g_q.capacity = g_q.slots.size();
+    g_q.running = true;
+    for (size_t i = 0; i < g_q.slots.size(); ++i) g_q.slots[i].worker = std::thread(worker_main, i);
+    return IMRESH_OK;
+}
+
+int queue_deinit()
+{
+    std::lock_guard<std::mutex> lk(g_init_mtx);
+    if (!g_q.running) return IMRESH_OK;
+    {
+        std::lock_guard<std::mutex> l2(g_q.mtx);
+        g_q.closing = true;
+    }
+    g_q.cv_work.notify_all();
+    for (auto &s : g_q.slots)
+        if (s.worker.joinable()) s.worker.join();
+    int st = IMRESH_OK;
+    for (auto &s : g_q.slots) {
+        hipError_t e = hipStreamDestroy(s.stream);
+        if (e != hipSuccess) st = imr::from_hip(e);
+    }
+    g_q.slots.clear();
+    g_q.running = false;
+    return st;
+}
+
+int queue_add(Task &&t)
+{
+    std::unique_lock<std::mutex> lk(g_q.mtx);
+    if (!g_q.running || g_q.closing) return IMRESH_ERR_NOT_INITIALIZED;
+    g_q.cv_space.wait(lk, [] { return g_q.closing || g_q.pending.size() < g_q.capacity; });
+    if (g_q.closing) return IMRESH_ERR_NOT_INITIALIZED;
+    g_q.pending.push_back(std::move(t));
+    lk.unlock();
+    g_q.cv_work.notify_one();
+    return IMRESH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int imresh_task_queue_init(int streams_per_device) { return queue_init(streams_per_device); }
+
+int imresh_task_queue_deinit(void) { return queue_deinit(); }
+
+int imresh_add_task(float *io_host, uint32_t width, uint32_t height, const imresh_params *params,
+                    imresh_task_callback callback, void *user)
+{
+    if (io_host == nullptr) return IMRESH_ERR_NULL;
+    if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
+    Task t;
+    t.io = io_host;
+    t.w = width;
+    t.h = height;
+    if (params) t.params = *params;
+    else {
+        imresh_params_default(&t.params);
+        t.params.defaults = IMRESH_DEFAULTS_QUEUE;
+    }
+    if (callback)
+        t.done = [callback, user](float *io, uint32_t w, uint32_t h, int st) { callback(user, io, w, h, st); };
+    return queue_add(std::move(t));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ C++ drop-in API
+namespace imresh {
+namespace algorithms {
+
+int shrinkWrap(float *const rIoData, unsigned int const rImageWidth, unsigned int const rImageHeight,
+               unsigned int rnCycles, float rTargetError, float rHioBeta,
+               float rIntensityCutOffAutoCorel, float rIntensityCutOff, float sigma0,
+               float rSigmaChange, unsigned int rnHioCycles)
+{
+    if (rIoData == nullptr) return 1;   // shrinkWrap.cpp:133
+    imresh_params p;
+    p.n_cycles = rnCycles;
+    p.target_error = rTargetError;
+    p.hio_beta = rHioBeta;
+    p.cutoff_autocorel = rIntensityCutOffAutoCorel;
+    p.cutoff = rIntensityCutOff;
+    p.sigma0 = sigma0;
+    p.sigma_change = rSigmaChange;
+    p.n_hio_cycles = rnHioCycles;
+    p.defaults = IMRESH_DEFAULTS_CPU;
+    return imresh_shrinkwrap(rIoData, rImageWidth, rImageHeight, &p, nullptr, nullptr);
+}
+
+}  // namespace algorithms
+
+namespace io {
+
+void taskQueueInit() { queue_init(0); }
+
+void taskQueueDeinit() { queue_deinit(); }
+
+void addTask(float *const ioData, unsigned int const imageWidth, unsigned int const imageHeight,
+             WriteOutCallBack writeOutFunc, std::string fileName, unsigned int nCycles,
+             unsigned int nHioCycles, float targetError, float hioBeta,
+             float intensityCutOffAutoCorel, float intensityCutOff, float sigma0, float sigmaChange)
+{
+    Task t;
+    t.io = ioData;
+    t.w = imageWidth;
+    t.h = imageHeight;
+    t.params.n_cycles = nCycles;
+    t.params.n_hio_cycles = nHioCycles;
+    t.params.target_error = targetError;
+    t.params.hio_beta = hioBeta;
+    t.params.cutoff_autocorel = intensityCutOffAutoCorel;
+    t.params.cutoff = intensityCutOff;
+    t.params.sigma0 = sigma0;
+    t.params.sigma_change = sigmaChange;
+    t.params.defaults = IMRESH_DEFAULTS_QUEUE;
+    t.done = [writeOutFunc, fileName](float *io, uint32_t w, uint32_t h, int) {
+        if (writeOutFunc) writeOutFunc(io, w, h, fileName);
+    };
+    queue_add(std::move(t));
+}
+
+}  // namespace io
+}  // namespace imresh

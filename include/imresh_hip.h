@@ -1,0 +1,190 @@
+/*
+ * imresh_hip.h -- C ABI of the MI355X (gfx950) shrink-wrap / HIO library.
+ *
+ * Plain C: pointers, sizes and an opaque HIP stream (hipStream_t passed as
+ * void*; NULL = the default stream).  Every function returns an int status
+ * (IMRESH_OK = 0); nothing calls exit()/abort() -- the reference printed and
+ * exited on any CUDA/cuFFT error (src/imresh/libs/cudacommon.cu:39-56,
+ * src/imresh/libs/checkCufftError.cpp:82-91).
+ *
+ * Memory convention: "dev" pointers are device (HBM) pointers, "host"
+ * pointers are ordinary host memory.  Complex arrays are interleaved float
+ * pairs (re, im), row-major H rows x W columns, DC at [0,0], matching
+ * cufftComplex / fftwf_complex in the reference.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef IMRESH_HIP_H
+#define IMRESH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMRESH_ABI_VERSION 1
+
+enum imresh_status {
+    IMRESH_OK = 0,
+    IMRESH_ERR_NULL = 1,             /* null pointer (shrinkWrap.cpp:133 returns 1) */
+    IMRESH_ERR_INVALID = 2,          /* invalid size / parameter                 */
+    IMRESH_ERR_UNSUPPORTED_SIZE = 3, /* no FFT plan for this length               */
+    IMRESH_ERR_OUT_OF_MEMORY = 4,
+    IMRESH_ERR_NO_DEVICE = 5,        /* taskQueue.cu:206-211 exited instead       */
+    IMRESH_ERR_HIP = 6,              /* any other HIP runtime error               */
+    IMRESH_ERR_KERNEL_TOO_LARGE = 7, /* Gaussian kernel > 63 taps (gaussian.cpp:189-192) */
+    IMRESH_ERR_NOT_INITIALIZED = 8,  /* addTask before taskQueueInit (taskQueue.cu:163) */
+    IMRESH_ERR_BUDGET = 9            /* request exceeds the configured HBM budget */
+};
+
+/* Default-substitution rule for parameters <= 0 (or == 0 for counts). */
+enum imresh_defaults {
+    /* shrinkWrap.cpp:133-140: n_cycles has no default (0 => no cycles),
+     * cutoff_autocorel default 0.04 */
+    IMRESH_DEFAULTS_CPU = 0,
+    /* cudaShrinkWrap.cu:193-200 (the taskQueue path): n_cycles 0 => 20,
+     * cutoff_autocorel default 0.02 */
+    IMRESH_DEFAULTS_QUEUE = 1
+};
+
+/* Parameters of one reconstruction; names and meaning follow
+ * shrinkWrap.hpp:40-53 / cudaShrinkWrap.hpp:114-130. */
+typedef struct imresh_params {
+    uint32_t n_cycles;        /* shrink-wrap cycles                        */
+    float    target_error;    /* early exit when HIO error < this          */
+    float    hio_beta;        /* HIO feedback beta                         */
+    float    cutoff_autocorel;/* mask cut-off of cycle 0 (rel. to max)     */
+    float    cutoff;          /* mask cut-off of later cycles              */
+    float    sigma0;          /* first Gaussian blur sigma                 */
+    float    sigma_change;    /* sigma <- max(1.5, (1-sigma_change)*sigma) */
+    uint32_t n_hio_cycles;    /* HIO iterations per cycle                  */
+    int32_t  defaults;        /* enum imresh_defaults                      */
+} imresh_params;
+
+/* Outcome of one reconstruction (read back from the device). */
+typedef struct imresh_result {
+    uint32_t cycles_run;      /* cycles executed (early exit aware)        */
+    float    last_error;      /* sqrtf(sum mask*|g'|)/sum mask of last cycle */
+    float    last_total;      /* sum mask*|g'| of the last executed cycle  */
+    float    last_masked;     /* sum mask of the last executed cycle       */
+} imresh_result;
+
+/* ---- library ----------------------------------------------------------- */
+int         imresh_abi_version(void);
+const char *imresh_strerror(int status);
+/* Fills the reference defaults (shrinkWrap.hpp:45-52) into *p. */
+int imresh_params_default(imresh_params *p);
+/* Number of visible GPUs (taskQueue.cu:203-211 used cudaGetDeviceCount). */
+int imresh_device_count(int *count);
+
+/* ---- device memory helpers (plumbing for callers without a runtime) ---- */
+int imresh_dev_alloc(void **dev, uint64_t bytes);
+int imresh_dev_free(void *dev);
+int imresh_memcpy_h2d(void *dev, const void *host, uint64_t bytes, void *stream);
+int imresh_memcpy_d2h(void *host, const void *dev, uint64_t bytes, void *stream);
+/* asynchronous device-to-device copy on `stream` */
+int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream);
+int imresh_stream_sync(void *stream);
+
+/* ---- whole reconstruction ---------------------------------------------- */
+/* Replaces imresh::algorithms::shrinkWrap (src/imresh/algorithms/shrinkWrap.cpp:111-361,
+ * decl shrinkWrap.hpp:40-53) and cudaShrinkWrap (src/imresh/algorithms/cuda/
+ * cudaShrinkWrap.cu:164-420).  io_host holds the Fourier modulus |F| (W*H
+ * floats) on entry and Re(g') on return, unnormalized exactly as the
+ * reference.  Synchronous.  result may be NULL. */
+int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
+                      const imresh_params *params, void *stream, imresh_result *result);
+
+/* Same algorithm with the modulus already resident in HBM: io_dev (W*H
+ * floats, device) is read as |F| and overwritten with Re(g').  Asynchronous
+ * on `stream` when result == NULL; with result != NULL it synchronizes the
+ * stream and reads the cycle count / error back. */
+int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
+                             const imresh_params *params, void *stream, imresh_result *result);
+
+/* HIO inner loop from a given device state, n_iter iterations
+ * (cudaShrinkWrap.cu:311-365 / shrinkWrap.cpp:260-319): gprime (complex, g'),
+ * gprev (complex), mask (uint8, 1 = outside support), modulus (float).
+ * Runs the fused row/column kernels the reconstruction uses. */
+int imresh_hio_iterations(float *gprime_dev, float *gprev_dev, const uint8_t *mask_dev,
+                          const float *modulus_dev, uint32_t width, uint32_t height,
+                          float hio_beta, uint32_t n_iter, void *stream);
+
+/* ---- per-operator entry points (device pointers, asynchronous) --------- */
+/* cudaKernelCopyToRealPart, cudaVectorElementwise.cu:75-95 */
+int imresh_copy_to_real_part(float *cplx_dev, const float *real_dev, uint32_t n, void *stream);
+/* cudaKernelCopyFromRealPart, cudaVectorElementwise.cu:98-117 */
+int imresh_copy_from_real_part(float *real_dev, const float *cplx_dev, uint32_t n, void *stream);
+/* cudaKernelComplexNormElementwise, cudaVectorElementwise.cu:120-141;
+ * complexNormElementwise, vectorElementwise.cpp:41-56 */
+int imresh_complex_norm(float *out_dev, const float *cplx_dev, uint32_t n, void *stream);
+/* cudaKernelApplyComplexModulus, cudaVectorElementwise.cu:168-195;
+ * applyComplexModulus, vectorElementwise.cpp:59-81.  out may equal in. */
+int imresh_apply_complex_modulus(float *out_dev, const float *in_dev, const float *modulus_dev,
+                                 uint32_t n, void *stream);
+/* cudaKernelApplyHioDomainConstraints, cudaVectorElementwise.cu:42-72 (float mask) */
+int imresh_hio_domain_constraints(float *gprev_dev, const float *gprime_dev, const float *mask_dev,
+                                  uint32_t n, float hio_beta, void *stream);
+/* cudaKernelCutOff, cudaVectorElementwise.cu:198-219 */
+int imresh_cutoff(float *data_dev, uint32_t n, float threshold, float lower, float upper,
+                  void *stream);
+/* cudaVectorMax/Min/Sum, cudaVectorReduce.cu:216-252 (host result, synchronous) */
+int imresh_vector_max(const float *data_dev, uint32_t n, float *result_host, void *stream);
+int imresh_vector_min(const float *data_dev, uint32_t n, float *result_host, void *stream);
+int imresh_vector_sum(const float *data_dev, uint32_t n, float *result_host, void *stream);
+/* cudaCalculateHioError, cudaVectorReduce.cu:357-403 (float mask) and the
+ * unsigned char mask instantiation (:480-503).  Returns sqrtf(total)/nmasked
+ * in *error_host; total / nmasked may be NULL.  Synchronous. */
+int imresh_calculate_hio_error(const float *gprime_dev, const float *mask_dev, uint32_t n,
+                               int invert_mask, float *error_host, float *total_host,
+                               float *nmasked_host, void *stream);
+int imresh_calculate_hio_error_u8(const float *gprime_dev, const uint8_t *mask_dev, uint32_t n,
+                                  int invert_mask, float *error_host, float *total_host,
+                                  float *nmasked_host, void *stream);
+/* calcGaussianKernel, calcGaussianKernel.cpp:42-91 (host).  Returns the tap
+ * count; writes the weights when weights != NULL and capacity suffices. */
+int imresh_gaussian_kernel(double sigma, float *weights_host, uint32_t capacity);
+/* cudaGaussianBlur / gaussianBlur (cudaGaussian.cu:1044-1057, gaussian.cpp:737-749),
+ * the horizontal pass (cudaGaussian.cu:655-726) and the vertical pass
+ * (cudaGaussian.cu:993-1041); in place, clamp-to-edge, up to 63 taps. */
+int imresh_gaussian_blur(float *data_dev, uint32_t width, uint32_t height, double sigma,
+                         void *stream);
+int imresh_gaussian_blur_horizontal(float *data_dev, uint32_t width, uint32_t height,
+                                    double sigma, void *stream);
+int imresh_gaussian_blur_vertical(float *data_dev, uint32_t width, uint32_t height,
+                                  double sigma, void *stream);
+/* cufftExecC2C on a cufftPlan2d(H, W, C2C) plan (cudaShrinkWrap.cu:225-232,319,343):
+ * unnormalized 2-D transform; direction -1 = FORWARD, +1 = INVERSE/BACKWARD.
+ * out may equal in. */
+int imresh_fft2d(float *out_dev, const float *in_dev, uint32_t width, uint32_t height,
+                 int direction, void *stream);
+/* 1 if an FFT plan exists for this line length, else 0. */
+int imresh_fft_supported(uint32_t n);
+
+/* ---- task queue (taskQueue.hpp:38-84, taskQueue.cu:91-251) --------------- */
+/* Completion callback: (user, io_host, width, height, status). */
+typedef void (*imresh_task_callback)(void *user, float *io_host, uint32_t width,
+                                     uint32_t height, int status);
+/* taskQueueInit, taskQueue.cu:231-234 (streams per device: 0 = default) */
+int imresh_task_queue_init(int streams_per_device);
+/* taskQueueDeinit, taskQueue.cu:236-251: waits for all tasks */
+int imresh_task_queue_deinit(void);
+/* addTask, taskQueue.cu:147-193: blocks while all slots are busy */
+int imresh_add_task(float *io_host, uint32_t width, uint32_t height,
+                    const imresh_params *params, imresh_task_callback callback, void *user);
+
+/* ---- measurement ------------------------------------------------------- */
+/* Average device duration (ms, HIP events on `stream`) of each fused kernel
+ * of one HIO iteration at W x H, over `reps` back-to-back launches on a
+ * resident synthetic state.  out_ms[0] = row pass (inverse+constraint+
+ * forward), out_ms[1] = column pass (forward+modulus+inverse),
+ * out_ms[2] = blur+max, out_ms[3] = row pass closing a cycle (norm+error). */
+int imresh_profile_kernels(uint32_t width, uint32_t height, uint32_t reps, float *out_ms,
+                           void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""GPU parity of the fused HIO path and of whole reconstructions.
+
+* one HIO iteration from the same state (fused row/column kernels, through
+  imresh_hio_iterations) vs the oracle: <= 1e-5 relative L2, against both the
+  float32 and the float64 oracle;
+* short reconstructions (1 cycle x 3 and 2 cycles x 2 iterations, including
+  the 2048^2 benchmark workload): object within 1e-4 of the fp32 oracle, HIO
+  error metric and NCC to the known object (modulo translation and twin)
+  within 1e-3 of the fp64 oracle;
+* full 20 x 20 reconstructions: HIO is chaotic in floating point (the fp32
+  and fp64 restatements of the reference end O(1) apart), so they are judged
+  on a converging workload (NCC -> 1 in every precision) and as an ensemble;
+* early exit, n_cycles = 0, the "<= 0 selects the default" rule and bitwise
+  determinism (including the full 2048^2 benchmark workload).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import imresh_amd as im
+import oracle
+from helpers import FLT_MIN
+from imresh_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+B = im.DeviceBuffer
+L = im.lib
+
+
+def _state(w, h, index=0, cut=0.04, sigma=3.0, shapes=8):
+    """A realistic state: g' = IFFT2(M) (autocorrelation), gPrev = g',
+    mask from the oracle's cycle-0 mask update."""
+    obj, M = syn.workload(w, h, n_shapes=shapes, central=min(w, h) // 2, index=index)
+    g = oracle.fft2d(M.astype(np.complex64), +1, np.float32)
+    mask = oracle.make_mask(g, sigma, cut, np.float32)
+    return obj, M, g, g.copy(), mask
+
+
+def _gpu_iterations(M, g, gp, mask, beta, n_iter):
+    h, w = M.shape
+    dg, dgp = B.from_array(g), B.from_array(gp)
+    dm, dM = B.from_array(mask.astype(np.uint8)), B.from_array(M)
+    im.check(L().imresh_hio_iterations(dg.ptr, dgp.ptr, dm.ptr, dM.ptr, w, h, np.float32(beta),
+                                       n_iter, None))
+    return dg.download(np.complex64, (h, w)), dgp.download(np.complex64, (h, w))
+
+
+@pytest.mark.parametrize("w,h", [(64, 64), (256, 256), (96, 200), (1024, 1024), (2048, 2048)])
+def test_one_hio_iteration_matches_oracle(w, h):
+    obj, M, g, gp, mask = _state(w, h, index=w + h)
+    g_gpu, gp_gpu = _gpu_iterations(M, g, gp, mask, 0.9, 1)
+    g32, gp32 = oracle.hio_iterations(g, gp, mask, M, 0.9, 1, np.float32)
+    g64, gp64 = oracle.hio_iterations(g.astype(np.complex128), gp.astype(np.complex128),
+                                      mask.astype(np.float64), M.astype(np.float64), 0.9, 1,
+                                      np.float64)
+    e32, e64 = syn.rel_l2(g_gpu, g32), syn.rel_l2(g_gpu, g64)
+    print(f"{w}x{h}: one iteration rel L2 vs fp32 oracle {e32:.3e}, vs fp64 oracle {e64:.3e}; "
+          f"fp32 oracle vs fp64 {syn.rel_l2(g32, g64):.3e}")
+    assert e32 < 1e-5 and e64 < 1e-5
+    # gPrev after the constraint is elementwise from identical inputs: exact
+    assert np.array_equal(gp_gpu.view(np.float32), gp32.view(np.float32))
+
+
+def test_twenty_iterations_track_fp64_like_fp32_oracle():
+    """HIO is chaotic in floating point: after 20 iterations the fp32 oracle
+    itself is ~1e-1 away from the fp64 oracle.  The GPU must not be further
+    from fp64 than the fp32 restatement of the reference is."""
+    obj, M, g, gp, mask = _state(256, 256, index=3)
+    g_gpu, _ = _gpu_iterations(M, g, gp, mask, 0.9, 20)
+    g32, _ = oracle.hio_iterations(g, gp, mask, M, 0.9, 20, np.float32)
+    g64, _ = oracle.hio_iterations(g.astype(np.complex128), gp.astype(np.complex128),
+                                   mask.astype(np.float64), M.astype(np.float64), 0.9, 20,
+                                   np.float64)
+    e_gpu, e_cpu = syn.rel_l2(g_gpu, g64), syn.rel_l2(g32, g64)
+    print(f"20 iterations: GPU vs fp64 {e_gpu:.3e}, fp32 oracle vs fp64 {e_cpu:.3e}")
+    assert e_gpu < 3 * e_cpu + 1e-6
+
+
+def _recon(M, dtype, **kw):
+    out, st = oracle.shrink_wrap(M, dtype=dtype, **kw)
+    return out, st
+
+
+SHORT = [("64", lambda: syn.workload(64, 64, n_shapes=8, central=32, index=1)),
+         ("256-config", lambda: syn.config_256(0)),
+         ("96x200", lambda: syn.workload(96, 200, n_shapes=8, central=48, index=4)),
+         ("2048-config", lambda: syn.config_2048(0))]
+
+
+@pytest.mark.parametrize("name,make", SHORT, ids=[s[0] for s in SHORT])
+@pytest.mark.parametrize("cycles,hio", [(1, 3), (2, 2)])
+def test_short_reconstruction_matches_oracle(name, make, cycles, hio):
+    """Runs short enough that fp32 rounding has not been amplified yet:
+    object within 1e-4 of the fp32 oracle, HIO error metric and NCC within
+    1e-3 of the fp64 oracle (the fp32 oracle sums the error sequentially in
+    float, like the reference's OpenMP reduction, which is itself ~1e-4 off
+    at 2048^2)."""
+    oracle.set_fft_threads(8)
+    obj, M = make()
+    kw = dict(n_cycles=cycles, target_error=FLT_MIN, n_hio_cycles=hio)
+    gpu = M.copy()
+    r = im.shrink_wrap(gpu, **kw)
+    c32, s32 = _recon(M, np.float32, **kw)
+    c64, s64 = _recon(M, np.float64, **kw)
+    assert r.cycles_run == s32.cycles_run == s64.cycles_run == cycles
+    d_obj = syn.rel_l2(gpu, c32)
+    rel = abs(r.last_error - s64.last_error) / abs(s64.last_error)
+    n_gpu, n_64 = syn.ncc(obj, gpu), syn.ncc(obj, c64)
+    print(f"{name} {cycles}x{hio}: object vs fp32 {d_obj:.2e} (fp32 vs fp64 {syn.rel_l2(c32, c64):.2e}); "
+          f"error gpu {r.last_error:.7g} fp64 {s64.last_error:.7g} fp32 {s32.last_error:.7g} "
+          f"(rel {rel:.2e}); masked {r.last_masked:.0f}/{s64.last_masked:.0f}; NCC {n_gpu:.6f}/{n_64:.6f}")
+    assert d_obj < 1e-4
+    assert r.last_masked == s64.last_masked
+    assert rel < 1e-3
+    assert abs(n_gpu - n_64) < 1e-3
+
+
+def test_converging_reconstruction_matches_oracle():
+    """A workload on which 20 x 20 shrink-wrap converges (NCC -> 1) in both
+    precisions: the end result is then insensitive to rounding."""
+    obj, M = syn.workload(64, 64, n_shapes=8, central=32, index=102)
+    kw = dict(n_cycles=20, target_error=FLT_MIN, n_hio_cycles=20)
+    gpu = M.copy()
+    r = im.shrink_wrap(gpu, **kw)
+    c32, s32 = _recon(M, np.float32, **kw)
+    c64, s64 = _recon(M, np.float64, **kw)
+    n = [syn.ncc(obj, x) for x in (gpu, c32, c64)]
+    print(f"converging 64x64: NCC gpu {n[0]:.6f} fp32 {n[1]:.6f} fp64 {n[2]:.6f}; error "
+          f"{r.last_error:.3g}/{s32.last_error:.3g}/{s64.last_error:.3g}")
+    assert min(n) > 0.999
+    assert abs(n[0] - n[2]) < 1e-3
+
+
+def test_full_reconstruction_ensemble():
+    """Full 20 x 20 reconstructions diverge chaotically from any other
+    floating-point evaluation (the fp32 and fp64 restatements of the
+    reference differ by O(1) on most seeds), so they are compared as an
+    ensemble: the GPU's mean NCC over 8 seeds must not fall below the fp32
+    reference restatement's mean by more than 0.15."""
+    kw = dict(n_cycles=20, target_error=FLT_MIN, n_hio_cycles=20)
+    rows = []
+    for s in range(8):
+        obj, M = syn.workload(64, 64, n_shapes=8, central=32, index=100 + s)
+        gpu = M.copy()
+        r = im.shrink_wrap(gpu, **kw)
+        c32, _ = _recon(M, np.float32, **kw)
+        c64, _ = _recon(M, np.float64, **kw)
+        rows.append((syn.ncc(obj, gpu), syn.ncc(obj, c32), syn.ncc(obj, c64)))
+        assert r.cycles_run == 20
+    a = np.array(rows)
+    print("NCC per seed (gpu, fp32, fp64):\n" + "\n".join(f"  {x:.4f} {y:.4f} {z:.4f}" for x, y, z in a))
+    print(f"mean: gpu {a[:, 0].mean():.4f} fp32 {a[:, 1].mean():.4f} fp64 {a[:, 2].mean():.4f}")
+    assert a[:, 0].mean() > a[:, 1].mean() - 0.15
+
+
+def test_early_exit_matches_oracle():
+    obj, M = syn.workload(64, 64, n_shapes=6, central=32, index=9)
+    kw = dict(n_cycles=20, target_error=1e6, n_hio_cycles=5)
+    gpu = M.copy()
+    r = im.shrink_wrap(gpu, **kw)
+    cpu, st = _recon(M, np.float32, **kw)
+    assert r.cycles_run == st.cycles_run == 1
+    assert syn.rel_l2(gpu, cpu) < 1e-4
+
+
+def test_zero_cycles_is_inverse_fft_real_part():
+    obj, M = syn.workload(128, 64, n_shapes=6, central=32, index=10)
+    gpu = M.copy()
+    r = im.shrink_wrap(gpu, n_cycles=0)
+    assert r.cycles_run == 0
+    exp = np.real(np.fft.ifft2(M.astype(np.float64))) * M.size
+    assert syn.rel_l2(gpu, exp) < 1e-6
+
+
+def test_defaults_rule():
+    """<= 0 selects the default (shrinkWrap.cpp:134-140)."""
+    obj, M = syn.workload(64, 64, n_shapes=6, central=32, index=11)
+    a, b = M.copy(), M.copy()
+    ra = im.shrink_wrap(a, n_cycles=3, target_error=0, hio_beta=0, cutoff_autocorel=0, cutoff=0,
+                        sigma0=0, sigma_change=0, n_hio_cycles=0)
+    rb = im.shrink_wrap(b, n_cycles=3)
+    assert np.array_equal(a, b) and ra.cycles_run == rb.cycles_run
+
+
+def test_deterministic_bitwise():
+    obj, M = syn.workload(256, 256, n_shapes=8, central=128, index=12)
+    a, b = M.copy(), M.copy()
+    im.shrink_wrap(a, n_cycles=4, target_error=FLT_MIN)
+    im.shrink_wrap(b, n_cycles=4, target_error=FLT_MIN)
+    assert np.array_equal(a, b)
+
+
+def test_null_io_returns_error():
+    p = im.default_params()
+    assert L().imresh_shrinkwrap(None, 8, 8, ctypes.byref(p), None, None) == 1
+
+
+def test_config_2048_full_run_deterministic():
+    """The benchmark workload (2048^2, 20 x 20) on the device-resident path:
+    bitwise identical across runs, finite, 20 cycles executed."""
+    obj, M = syn.config_2048(0)
+    d = B.from_array(M)
+    p = im.default_params(n_cycles=20, target_error=FLT_MIN)
+    outs = []
+    for _ in range(2):
+        d.upload(M)
+        res = im.shrink_wrap_device(d.ptr, 2048, 2048, p)
+        assert res.cycles_run == 20
+        outs.append(d.download(np.float32, M.shape))
+    assert np.array_equal(outs[0], outs[1])
+    assert np.all(np.isfinite(outs[0]))
+    print(f"2048^2 20x20: final error {res.last_error:.6g}, NCC to object {syn.ncc(obj, outs[0]):.4f}")

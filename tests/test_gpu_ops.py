@@ -1,0 +1,325 @@
+"""GPU parity of the per-operator entry points against the CPU oracle.
+
+Elementwise operators, blur, max/min and the cut-off are bit-exact given
+identical inputs (the library rounds exactly as the reference's C++); sums
+are checked against a float64 sum (1e-6 relative); the FFT against numpy's
+float64 pocketfft.  Known-answer tests follow the reference's own
+(testVectorReduce.cu, testGaussian.cpp).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import imresh_amd as im
+import oracle
+from helpers import FLT_EPS, KAT_SEED, RAND_MAX, compare_float_array, log_spaced_points, rand, srand
+
+pytestmark = pytest.mark.gpu
+B = im.DeviceBuffer
+L = im.lib
+
+
+def _f(x):
+    return ctypes.byref(x)
+
+
+def run_norm(c):
+    n = c.size
+    dc, do = B.from_array(c), B(4 * n)
+    im.check(L().imresh_complex_norm(do.ptr, dc.ptr, n, None))
+    return do.download(np.float32, c.shape)
+
+
+def test_complex_norm_bit_exact():
+    rng = np.random.default_rng(1)
+    n = 100003
+    c = (rng.standard_normal(n) * 10 + 1j * rng.standard_normal(n)).astype(np.complex64)
+    c[::17] = 0
+    c[5] = complex(3, 4)
+    got = run_norm(c)
+    assert np.array_equal(got, oracle.complex_norm(c))
+    assert got[5] == 5.0 and got[0] == 0.0
+
+
+def test_apply_modulus_bit_exact_and_zero():
+    rng = np.random.default_rng(2)
+    n = 65536 + 7
+    c = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    c[::13] = 0
+    M = rng.random(n, dtype=np.float32) * 5
+    dc, dM = B.from_array(c), B.from_array(M)
+    im.check(L().imresh_apply_complex_modulus(dc.ptr, dc.ptr, dM.ptr, n, None))
+    got = dc.download(np.complex64, n)
+    exp = oracle.apply_modulus(c, M)
+    assert np.array_equal(got.view(np.float32), exp.view(np.float32))
+    assert np.all(got[::13] == 0)          # norm == 0 -> norm = 1 -> stays 0
+    r = oracle.ref()
+    if r is not None:
+        ci = c.view(np.float32).copy()
+        r.ref_apply_modulus_f32(ci.ctypes.data, M.ctypes.data, n)
+        assert np.array_equal(got.view(np.float32), ci)
+
+
+def test_hio_constraint_bit_exact():
+    rng = np.random.default_rng(3)
+    n = 50001
+    g = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    gp = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    mask = (rng.random(n) < 0.4).astype(np.float32)
+    gr = g.view(np.float32).reshape(-1, 2)
+    gr[::11, 0] = -0.0                      # Re g' = -0.0: "< 0" is false
+    dgp, dg, dm = B.from_array(gp), B.from_array(g), B.from_array(mask)
+    im.check(L().imresh_hio_domain_constraints(dgp.ptr, dg.ptr, dm.ptr, n, np.float32(0.9), None))
+    got = dgp.download(np.complex64, n)
+    exp = oracle.hio_constraint(gp, g, mask, 0.9)
+    assert np.array_equal(got.view(np.float32), exp.view(np.float32))
+
+
+def test_cutoff_strict_less_bit_exact():
+    rng = np.random.default_rng(4)
+    n = 40000
+    d = rng.random(n, dtype=np.float32)
+    t = d[123]                              # values equal to the threshold map to upper
+    dd = B.from_array(d)
+    im.check(L().imresh_cutoff(dd.ptr, n, t, 1.0, 0.0, None))
+    got = dd.download(np.float32, n)
+    assert np.array_equal(got, oracle.cutoff(d, t, 1.0, 0.0))
+    assert got[123] == 0.0
+
+
+def test_copy_real_part_roundtrip():
+    rng = np.random.default_rng(5)
+    n = 1000
+    x = rng.standard_normal(n).astype(np.float32)
+    dx, dc = B.from_array(x), B(8 * n)
+    im.check(L().imresh_copy_to_real_part(dc.ptr, dx.ptr, n, None))
+    c = dc.download(np.complex64, n)
+    assert np.array_equal(c.real, x) and np.all(c.imag == 0)
+    dy = B(4 * n)
+    im.check(L().imresh_copy_from_real_part(dy.ptr, dc.ptr, n, None))
+    assert np.array_equal(dy.download(np.float32, n), x)
+
+
+# ------------------------------------------------------------ reductions
+def _reduce(name, d):
+    dd = B.from_array(d)
+    out = ctypes.c_float()
+    im.check(getattr(L(), name)(dd.ptr, d.size, _f(out), None), name)
+    return np.float32(out.value)
+
+
+def test_reduce_length_one():
+    """testVectorReduce.cu:93-99"""
+    x = np.array([0.1234567], dtype=np.float32)
+    for name in ("imresh_vector_max", "imresh_vector_min", "imresh_vector_sum"):
+        assert _reduce(name, x) == x[0]
+
+
+def test_reduce_obvious_value_kat():
+    """testVectorReduce.cu:83-187: random U[-0.5, 0.5) data (srand 350471643),
+    one slot set to +-7.37519 at rand() % n; max/min must be exact."""
+    nmax = 1 << 22
+    srand(KAT_SEED)
+    data = np.empty(nmax, dtype=np.float32)
+    # (float) rand() / RAND_MAX - 0.5f, vectorised over the glibc stream
+    raw = np.array([rand() for _ in range(nmax)], dtype=np.float64)
+    data[:] = (raw.astype(np.float32) / np.float32(RAND_MAX)).astype(np.float32) - np.float32(0.5)
+    dd = B.from_array(data)
+    for n in log_spaced_points(2, nmax, 30):
+        pos = rand() % n
+        for v, name in ((7.37519, "imresh_vector_max"), (-7.37519, "imresh_vector_min")):
+            saved = data[pos]
+            data[pos] = np.float32(v)
+            dd.upload(data[:n])
+            out = ctypes.c_float()
+            im.check(getattr(L(), name)(dd.ptr, n, _f(out), None))
+            assert np.float32(out.value) == np.float32(v), (n, name)
+            assert np.float32(out.value) == (oracle.vmax(data[:n]) if v > 0 else oracle.vmin(data[:n]))
+            data[pos] = saved
+
+
+def test_reduce_max_min_exact_random():
+    rng = np.random.default_rng(6)
+    for n in (1, 63, 64, 65, 1000, 123457, 1 << 20):
+        d = (rng.standard_normal(n) * 100).astype(np.float32)
+        assert _reduce("imresh_vector_max", d) == oracle.vmax(d)
+        assert _reduce("imresh_vector_min", d) == oracle.vmin(d)
+
+
+def test_reduce_sum_within_1e6():
+    rng = np.random.default_rng(7)
+    for n in (1, 1000, 1 << 20, 3 << 20):
+        d = rng.random(n, dtype=np.float32)
+        exact = float(np.sum(d.astype(np.float64)))
+        got = float(_reduce("imresh_vector_sum", d))
+        assert abs(got - exact) <= 1e-6 * abs(exact), (n, got, exact)
+
+
+# ------------------------------------------------------------ HIO error
+def _kat_mask_and_data(nmax):
+    """testVectorReduce.cu:297-338: random bit mask (MSB first per 32-bit
+    word, srand(350471643)); masked pixels = 3 + 4i, others random."""
+    srand(KAT_SEED)
+    nwords = (nmax + 31) // 32
+    words = np.array([rand() % 0xFFFFFFFF for _ in range(nwords)], dtype=np.uint64)
+    bits = ((words[:, None] >> np.arange(31, -1, -1, dtype=np.uint64)[None, :]) & 1).astype(np.float32)
+    mask = bits.reshape(-1)[:nmax].copy()
+    data = np.empty(nmax, dtype=np.complex64)
+    for i in range(nmax):
+        if mask[i]:
+            data[i] = complex(3.0, 4.0)
+        else:
+            re = np.float32(rand()) / np.float32(RAND_MAX)
+            im_ = np.float32(rand()) / np.float32(RAND_MAX)
+            data[i] = complex(re, im_)
+    return mask, data
+
+
+def test_hio_error_pythagorean_kat():
+    nmax = 1 << 18
+    mask, data = _kat_mask_and_data(nmax)
+    dg, dmf, dmu = B.from_array(data), B.from_array(mask), B.from_array(mask.astype(np.uint8))
+    last = 0.0
+    for n in log_spaced_points(2, nmax, 30):
+        for fn, dm in (("imresh_calculate_hio_error", dmf), ("imresh_calculate_hio_error_u8", dmu)):
+            err, tot, nm = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+            im.check(getattr(L(), fn)(dg.ptr, dm.ptr, n, 0, _f(err), _f(tot), _f(nm), None))
+            if tot.value < 16777216:
+                assert nm.value * 5 == tot.value, (n, fn)
+                assert nm.value == float(mask[:n].sum())
+                assert nm.value >= last
+            e_cpu, t_cpu, n_cpu = oracle.hio_error(data[:n], mask[:n])
+            assert nm.value == n_cpu
+            assert abs(tot.value - t_cpu) <= np.sqrt(n) * FLT_EPS * max(abs(t_cpu), 1e-30)
+        last = nm.value
+
+
+def test_hio_error_random_vs_float64():
+    rng = np.random.default_rng(8)
+    n = 1 << 20
+    g = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    m = (rng.random(n) < 0.3).astype(np.float32)
+    for invert in (0, 1):
+        err, tot, nm = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        dg, dm = B.from_array(g), B.from_array(m)
+        im.check(L().imresh_calculate_hio_error(dg.ptr, dm.ptr, n, invert, _f(err), _f(tot), _f(nm), None))
+        w = (1 - m) if invert else m
+        exact = float(np.sum(w.astype(np.float64) * np.abs(g.astype(np.complex128))))
+        assert abs(tot.value - exact) <= 1e-6 * exact
+        assert nm.value == float(w.sum())
+        assert abs(err.value - np.float32(np.sqrt(np.float32(tot.value))) / nm.value) == 0
+
+
+# ------------------------------------------------------------ blur
+BLUR_SIZES = [(1, 1), (1, 5), (5, 1), (7, 3), (37, 31), (234, 512), (100, 1021), (333, 64)]
+BLUR_SIGMAS = [0.1, 0.5, 1.0, 1.3, 1.7, 2.0, 3.0, 4.0, 6.0, 6.4, 10.0]
+
+
+@pytest.mark.parametrize("h,w", BLUR_SIZES)
+def test_blur_bit_exact_vs_oracle(h, w):
+    rng = np.random.default_rng(h * 1000 + w)
+    d = (rng.random((h, w), dtype=np.float32) - 0.5).astype(np.float32)
+    dd = B(d.nbytes)
+    for s in BLUR_SIGMAS:
+        for which, fn in (("h", "imresh_gaussian_blur_horizontal"),
+                          ("v", "imresh_gaussian_blur_vertical"), ("hv", "imresh_gaussian_blur")):
+            dd.upload(d)
+            im.check(getattr(L(), fn)(dd.ptr, w, h, s, None), fn)
+            got = dd.download(np.float32, (h, w))
+            exp = oracle.blur(d, s, which)
+            assert np.array_equal(got, exp), (h, w, s, which, float(np.abs(got - exp).max()))
+
+
+def test_blur_dirac_deltas_kat():
+    """testGaussian.cpp:217-337: spaced deltas reproduce the weights."""
+    for s in (0.1, 0.5, 1, 1.3, 1.7, 2, 3, 4):
+        w = im.gaussian_kernel(s)
+        k, r = len(w), (len(w) - 1) // 2
+        for ncols, nrows in ((31, 5), (234, 10), (1021, 3)):
+            if ncols < k:
+                continue
+            d = np.zeros((nrows, ncols), dtype=np.float32)
+            d[:, r:ncols - r:k] = 1
+            sol = np.zeros_like(d)
+            c = 0
+            while c + k - 1 < ncols:
+                sol[:, c:c + k] = w
+                c += k
+            dd = B.from_array(d)
+            im.check(L().imresh_gaussian_blur_horizontal(dd.ptr, ncols, nrows, s, None))
+            got = dd.download(np.float32, d.shape)
+            assert compare_float_array(got, sol) <= FLT_EPS * 30
+            # vertical on the transpose
+            dt = np.ascontiguousarray(d.T)
+            dd = B.from_array(dt)
+            im.check(L().imresh_gaussian_blur_vertical(dd.ptr, nrows, ncols, s, None))
+            got = dd.download(np.float32, dt.shape)
+            assert compare_float_array(got, np.ascontiguousarray(sol.T)) <= FLT_EPS * 30
+
+
+def test_blur_constant_rows_identical():
+    """testGaussian.cpp:400-544: a constant stays constant, all outputs identical."""
+    for s in (0.5, 1.7, 3, 4):
+        d = np.full((37, 513), 7.37519, dtype=np.float32)
+        dd = B.from_array(d)
+        im.check(L().imresh_gaussian_blur(dd.ptr, 513, 37, s, None))
+        got = dd.download(np.float32, d.shape)
+        assert np.all(got == got.flat[0])
+        assert compare_float_array(got, d) <= FLT_EPS * 30
+
+
+def test_blur_too_wide_kernel_is_an_error():
+    d = B(4 * 64)
+    st = L().imresh_gaussian_blur(d.ptr, 8, 8, 11.0, None)   # 65 taps
+    assert st == 7  # IMRESH_ERR_KERNEL_TOO_LARGE
+
+
+# ------------------------------------------------------------ FFT
+FFT_SIZES = [(1, 1), (2, 4), (8, 8), (3, 5), (12, 60), (16, 32), (64, 64), (128, 256),
+             (96, 200), (384, 768), (512, 512), (1024, 64), (2048, 16), (16, 2048),
+             (4096, 8), (8, 8192)]
+
+
+def _fft_gpu(x, direction):
+    h, w = x.shape
+    d = B.from_array(x.astype(np.complex64))
+    im.check(L().imresh_fft2d(d.ptr, d.ptr, w, h, direction, None), f"fft {w}x{h}")
+    return d.download(np.complex64, (h, w))
+
+
+@pytest.mark.parametrize("h,w", FFT_SIZES)
+def test_fft2d_vs_numpy(h, w):
+    rng = np.random.default_rng(h * 7 + w)
+    x = (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)
+    x64 = x.astype(np.complex128)
+    fwd = _fft_gpu(x, -1)
+    exp = np.fft.fft2(x64)
+    tol = 2e-7 * max(1.0, np.log2(h * w))
+    assert np.linalg.norm(fwd - exp) / np.linalg.norm(exp) < tol
+    inv = _fft_gpu(x, +1)
+    exp = np.fft.ifft2(x64) * (h * w)
+    assert np.linalg.norm(inv - exp) / np.linalg.norm(exp) < tol
+
+
+def test_fft2d_closed_forms():
+    h, w = 64, 128
+    d = np.zeros((h, w), np.complex64)
+    d[0, 0] = 1
+    assert np.allclose(_fft_gpu(d, -1), 1, atol=0)          # delta -> all ones, exactly
+    c = np.full((h, w), 0.5, np.complex64)
+    F = _fft_gpu(c, -1)
+    assert F[0, 0] == 0.5 * h * w and np.abs(F).ravel()[1:].max() < 1e-3
+    ky, kx = 5, 17
+    yy, xx = np.mgrid[0:h, 0:w]
+    e = np.exp(2j * np.pi * (ky * yy / h + kx * xx / w)).astype(np.complex64)
+    F = _fft_gpu(e, -1)
+    assert abs(F[ky, kx] - h * w) < 1e-2 * h * w
+    F[ky, kx] = 0
+    assert np.abs(F).max() < 1e-3 * h * w
+
+
+def test_fft_unsupported_size_is_an_error():
+    d = B(8 * 7 * 7)
+    assert L().imresh_fft2d(d.ptr, d.ptr, 7, 7, -1, None) == 3   # IMRESH_ERR_UNSUPPORTED_SIZE
+    assert L().imresh_fft_supported(2048) == 1 and L().imresh_fft_supported(7) == 0
