@@ -120,12 +120,10 @@ def _real(a, dtype):
 
 
 def _cplx_view(a, dtype):
-    """complex ndarray -> interleaved real ndarray of `dtype` (copy)."""
-    a = np.asarray(a)
-    if np.iscomplexobj(a):
-        ct = np.complex64 if np.dtype(dtype) == np.float32 else np.complex128
-        return np.ascontiguousarray(a.astype(ct)).view(dtype)
-    return _real(a, dtype)
+    """any ndarray -> interleaved (re, im) array of `dtype`, a fresh copy
+    (real input gets a zero imaginary part)."""
+    ct = np.complex64 if np.dtype(dtype) == np.float32 else np.complex128
+    return np.array(np.asarray(a), dtype=ct, order="C", copy=True).view(dtype)
 
 
 def _to_complex(a, dtype, shape):

@@ -1,0 +1,353 @@
+"""Pins of the CPU oracle (runs on the CPU, no GPU).
+
+Each oracle function is checked against something independent of its own
+formula: the reference's own CPU sources compiled into oracle/_ref (bit-exact
+where the reference is deterministic), the known-answer tests of the
+reference's test suite (testGaussian.cpp, testVectorReduce.cu), closed forms,
+float64 brute force, numpy's pocketfft, and committed golden fixtures
+(tests/golden/).  A plausible mistake -- a dropped tap, a wrong sign, a
+transposed index, a wrong clamp, '<=' for '<' -- fails at least one of them.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import FLT_EPS, KAT_SEED, RAND_MAX, compare_float_array, log_spaced_points, rand, srand
+
+REF = oracle.ref()
+needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference sources absent)")
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "oracle_golden.npz")
+
+
+# ------------------------------------------------------------ Gaussian kernel
+@pytest.mark.parametrize("sigma,taps", [(0.1, 1), (0.5, 3), (1.0, 7), (3.0, 19), (4.0, 25),
+                                        (6.0, 35), (6.4, 37), (10.9, 63), (11.0, 65)])
+def test_gauss_kernel_size_closed_form(sigma, taps):
+    """calcGaussianKernel.cpp:63: radius ceil(2.884402748387961466*sigma - 0.5)."""
+    w = oracle.gauss_kernel(sigma)
+    assert len(w) == taps == 2 * math.ceil(2.884402748387961466 * sigma - 0.5) + 1
+
+
+@pytest.mark.parametrize("sigma", [0.1, 0.5, 1.0, 1.3, 1.5, 1.7, 2.0, 2.9403, 3.0, 4.0, 6.0, 6.4])
+def test_gauss_kernel_properties(sigma):
+    w = oracle.gauss_kernel(sigma).astype(np.float64)
+    r = (len(w) - 1) // 2
+    assert abs(w.sum() - 1.0) < 1e-6
+    assert np.array_equal(w, w[::-1])
+    assert np.argmax(w) == r
+    # shape: ratio of neighbours follows exp(-(2i+1)/(2 sigma^2)) (to float rounding)
+    if r >= 2:
+        ratio = w[r + 1] / w[r]
+        assert abs(ratio - math.exp(-1.0 / (2 * sigma * sigma))) < 1e-6
+
+
+@needs_ref
+@pytest.mark.parametrize("sigma", [0.1, 0.5, 1.0, 1.3, 1.7, 2.0, 3.0, 4.0, 6.0, 6.4, 10.0])
+def test_gauss_kernel_bit_exact_vs_reference(sigma):
+    w = oracle.gauss_kernel(sigma)
+    buf = np.zeros(64, np.float32)
+    n = REF.ref_gauss_kernel_f32(sigma, buf.ctypes.data, 64)
+    assert n == len(w) and np.array_equal(w, buf[:n])
+
+
+def test_sigma_schedule_closed_form():
+    """sigma <- max(1.5, (1 - 0.01) * sigma) in float (shrinkWrap.cpp:245)."""
+    s = oracle.sigma_schedule(3.0, 0.01, 100)
+    exp = [np.float32(3.0)]
+    for _ in range(99):
+        exp.append(max(np.float32(1.5), np.float32(np.float32(0.99) * exp[-1])))
+    assert np.array_equal(s, np.array(exp, np.float32))
+    assert s[0] == 3.0 and s[-1] == 1.5 and np.all(np.diff(s) <= 0)
+    # 3 * 0.99^n >= 1.5 until n = 69 (reference note cudaGaussian.cu:84-86)
+    assert s[68] > 1.5 and s[70] == 1.5
+
+
+# ------------------------------------------------------------ blur
+@needs_ref
+@pytest.mark.parametrize("h,w", [(1, 1), (1, 7), (9, 1), (7, 3), (37, 31), (64, 1021), (500, 16)])
+def test_blur_bit_exact_vs_reference(h, w):
+    rng = np.random.default_rng(h * 31 + w)
+    d = (rng.random((h, w), dtype=np.float32) - 0.5).astype(np.float32)
+    for s in (0.1, 0.5, 1.0, 1.7, 3.0, 4.0, 6.0, 10.0):
+        for which, fn in (("h", REF.ref_blur_h_f32), ("v", REF.ref_blur_v_f32), ("hv", REF.ref_blur_f32)):
+            a = oracle.blur(d, s, which)
+            b = d.copy()
+            fn(b.ctypes.data, w, h, s)
+            assert np.array_equal(a, b), (h, w, s, which)
+
+
+def _brute_blur_rows(d, w):
+    """float64 direct convolution with clamp-to-edge along the last axis."""
+    r = (len(w) - 1) // 2
+    n = d.shape[-1]
+    idx = np.clip(np.arange(n)[:, None] + np.arange(-r, r + 1)[None, :], 0, n - 1)
+    return (d[..., idx] * w.astype(np.float64)).sum(-1)
+
+
+@pytest.mark.parametrize("sigma", [0.5, 1.3, 3.0, 6.4])
+def test_blur_vs_float64_brute_force(sigma):
+    rng = np.random.default_rng(11)
+    d = rng.random((40, 57), dtype=np.float32)
+    w = oracle.gauss_kernel(sigma)
+    h = _brute_blur_rows(d.astype(np.float64), w)
+    hv = _brute_blur_rows(h.T, w).T
+    for which, exp in (("h", h), ("v", _brute_blur_rows(d.astype(np.float64).T, w).T), ("hv", hv)):
+        got = oracle.blur(d, sigma, which)
+        assert compare_float_array(got, exp) <= 30 * FLT_EPS, which
+
+
+def test_blur_dirac_deltas_kat():
+    """testGaussian.cpp:217-337: spaced deltas reproduce the weight vector."""
+    for s in (0.1, 0.5, 1, 1.3, 1.7, 2, 3, 4):
+        w = oracle.gauss_kernel(s)
+        k, r = len(w), (len(w) - 1) // 2
+        for ncols in (1, 2, 3, 5, 10, 31, 37, 234):
+            if ncols < k:
+                continue
+            d = np.zeros((3, ncols), np.float32)
+            d[:, r:ncols - r:k] = 1
+            sol = np.zeros_like(d)
+            c = 0
+            while c + k - 1 < ncols:
+                sol[:, c:c + k] = w
+                c += k
+            assert compare_float_array(oracle.blur(d, s, "h"), sol) <= FLT_EPS * 30
+            assert compare_float_array(oracle.blur(np.ascontiguousarray(d.T), s, "v"),
+                                       np.ascontiguousarray(sol.T)) <= FLT_EPS * 30
+
+
+def test_blur_single_element_and_constants():
+    """testGaussian.cpp:340-544."""
+    srand(KAT_SEED)
+    for s in (0.1, 0.5, 1, 1.3, 1.7, 2, 3, 4):
+        x = np.array([[np.float32(rand()) / np.float32(RAND_MAX) - np.float32(0.5)]], np.float32)
+        assert compare_float_array(oracle.blur(x, s, "h"), x) <= FLT_EPS * 30
+        c = np.full((31, 513), 7.37519, np.float32)
+        for which in ("h", "v", "hv"):
+            b = oracle.blur(c, s, which)
+            assert np.all(b == b.flat[0])
+            assert compare_float_array(b, c) <= FLT_EPS * 30
+
+
+def test_blur_bound_invariants():
+    """testGaussian.cpp:134-148: min does not decrease, max does not increase,
+    max neighbour difference does not grow."""
+    srand(KAT_SEED)
+    n = 200 * 150
+    d = ((np.array([rand() for _ in range(n)], np.float64) / RAND_MAX) - 0.5).astype(np.float32)
+    d = d.reshape(200, 150)
+    b = oracle.blur(d, 3.0, "h")
+    for row_b, row_d in zip(b, d):
+        assert row_d.min() <= row_b.min() and row_d.max() >= row_b.max()
+        assert np.abs(np.diff(row_b)).max() <= np.abs(np.diff(row_d)).max()
+
+
+# ------------------------------------------------------------ reductions
+def test_reduce_obvious_value_kat():
+    """testVectorReduce.cu:83-187 on the oracle (exact)."""
+    srand(KAT_SEED)
+    nmax = 1 << 16
+    data = ((np.array([rand() for _ in range(nmax)], np.float64) / RAND_MAX) - 0.5).astype(np.float32)
+    assert oracle.vmax(data[:1]) == data[0] == oracle.vmin(data[:1]) == oracle.vsum(data[:1])
+    for n in log_spaced_points(2, nmax, 30):
+        pos = rand() % n
+        sub = data[:n].copy()
+        sub[pos] = np.float32(7.37519)
+        assert oracle.vmax(sub) == np.float32(7.37519)
+        sub[pos] = np.float32(-7.37519)
+        assert oracle.vmin(sub) == np.float32(-7.37519)
+
+
+@needs_ref
+def test_reduce_vs_reference():
+    rng = np.random.default_rng(3)
+    for n in (1, 17, 1000, 100003):
+        d = (rng.standard_normal(n) * 3).astype(np.float32)
+        assert oracle.vmax(d) == REF.ref_vmax_f32(d.ctypes.data, n)
+        assert oracle.vmin(d) == REF.ref_vmin_f32(d.ctypes.data, n)
+        s = float(np.sum(d.astype(np.float64)))
+        assert abs(oracle.vsum(d) - s) <= n * FLT_EPS * float(np.sum(np.abs(d)))
+
+
+# ------------------------------------------------------------ HIO error
+def test_hio_error_pythagorean_kat():
+    """testVectorReduce.cu:297-413: masked pixels = 3+4i => total = 5*nmasked."""
+    srand(KAT_SEED)
+    nmax = 1 << 15
+    words = np.array([rand() % 0xFFFFFFFF for _ in range((nmax + 31) // 32)], np.uint64)
+    bits = ((words[:, None] >> np.arange(31, -1, -1, dtype=np.uint64)[None, :]) & 1)
+    mask = bits.reshape(-1)[:nmax].astype(np.float32)
+    g = np.empty(nmax, np.complex64)
+    for i in range(nmax):
+        g[i] = complex(3, 4) if mask[i] else complex(np.float32(rand()) / np.float32(RAND_MAX),
+                                                     np.float32(rand()) / np.float32(RAND_MAX))
+    for n in log_spaced_points(2, nmax, 25):
+        e, tot, nm = oracle.hio_error(g[:n], mask[:n])
+        assert nm * 5 == tot and nm == mask[:n].sum()
+        assert e == np.float32(np.sqrt(np.float32(tot))) / np.float32(nm) or (nm == 0 and np.isnan(e))
+        e2, tot2, nm2 = oracle.hio_error(g[:n], mask[:n], invert=True)
+        assert nm2 == n - nm
+
+
+@needs_ref
+def test_hio_error_vs_reference():
+    import ctypes
+    rng = np.random.default_rng(5)
+    n = 100000
+    g = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    m = (rng.random(n) < 0.3).astype(np.float32)
+    gi = g.view(np.float32).copy()
+    tot, nm = ctypes.c_float(), ctypes.c_float()
+    REF.ref_hio_error_f32(gi.ctypes.data, m.ctypes.data, n, 0, ctypes.byref(tot), ctypes.byref(nm))
+    e, t, c = oracle.hio_error(g, m)
+    assert c == nm.value
+    assert abs(t - tot.value) <= math.sqrt(n) * FLT_EPS * abs(tot.value) * 10
+
+
+# ------------------------------------------------------------ elementwise
+@needs_ref
+def test_norm_and_modulus_bit_exact_vs_reference():
+    rng = np.random.default_rng(6)
+    n = 50000
+    c = (rng.standard_normal(n) * 7 + 1j * rng.standard_normal(n)).astype(np.complex64)
+    c[::9] = 0
+    out = np.empty(n, np.float32)
+    ci = c.view(np.float32).copy()
+    REF.ref_complex_norm_f32(out.ctypes.data, ci.ctypes.data, n)
+    assert np.array_equal(out, oracle.complex_norm(c))
+    M = rng.random(n, dtype=np.float32)
+    REF.ref_apply_modulus_f32(ci.ctypes.data, M.ctypes.data, n)
+    assert np.array_equal(ci, oracle.apply_modulus(c, M).view(np.float32))
+
+
+def test_modulus_projection_closed_form():
+    c = np.array([3 + 4j, 0, -6 + 8j, 1e-3j], np.complex64)
+    M = np.array([10, 5, 5, 2], np.float32)
+    out = oracle.apply_modulus(c, M)
+    assert out[0] == 6 + 8j and out[1] == 0 and out[2] == -3 + 4j and out[3] == 2j
+
+
+def test_hio_constraint_closed_form():
+    """cudaVectorElementwise.cu:61-70: feedback where mask == 1 or Re g' < 0."""
+    g = np.array([1 + 2j, -1 + 2j, 1 + 2j, complex(-0.0, 5)], np.complex64)
+    gp = np.array([10 + 10j] * 4, np.complex64)
+    mask = np.array([0, 0, 1, 0], np.float32)
+    out = oracle.hio_constraint(gp, g, mask, 0.5)
+    assert out[0] == 1 + 2j                     # inside support, Re >= 0: copy
+    assert out[1] == 10.5 + 9j                  # Re < 0: gPrev - beta*g'
+    assert out[2] == 9.5 + 9j                   # masked: gPrev - beta*g'
+    assert out[3] == complex(-0.0, 5)           # Re = -0.0 is not < 0: copy
+
+
+def test_cutoff_is_strict():
+    d = np.array([0.0, 0.5, 0.49999997, 1.0], np.float32)
+    assert np.array_equal(oracle.cutoff(d, np.float32(0.5), 1, 0), np.array([1, 0, 1, 0], np.float32))
+
+
+# ------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 12, 16, 60, 64, 96, 97, 128, 200, 243, 1021,
+                               1031, 2048])
+def test_fft_vs_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    s = np.abs(ref).max()
+    assert np.abs(oracle.fft(x, -1, np.float64) - ref).max() <= 1e-12 * s
+    assert np.abs(oracle.fft(x, +1, np.float64) - np.fft.ifft(x) * n).max() <= 1e-12 * s
+    assert np.linalg.norm(oracle.fft(x, -1, np.float32) - ref) <= 2e-6 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 13, 16, 24, 64])
+def test_fft_vs_brute_force_dft(n):
+    rng = np.random.default_rng(100 + n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    k = np.arange(n)
+    dft = np.exp(-2j * np.pi * np.outer(k, k) / n) @ x
+    assert np.abs(oracle.fft(x, -1, np.float64) - dft).max() <= 1e-12 * max(1, np.abs(dft).max())
+
+
+def test_fft2d_closed_forms_and_identities():
+    h, w = 24, 40
+    d = np.zeros((h, w), np.complex128)
+    d[0, 0] = 1
+    assert np.allclose(oracle.fft2d(d, -1, np.float64), 1, atol=1e-15)
+    assert np.allclose(oracle.fft2d(np.full((h, w), 2.0), -1, np.float64)[0, 0], 2 * h * w)
+    ky, kx = 3, 7
+    yy, xx = np.mgrid[0:h, 0:w]
+    e = np.exp(2j * np.pi * (ky * yy / h + kx * xx / w))
+    F = oracle.fft2d(e, -1, np.float64)
+    assert abs(F[ky, kx] - h * w) < 1e-9 and np.abs(np.delete(F.ravel(), ky * w + kx)).max() < 1e-9
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))
+    y = rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))
+    X = oracle.fft2d(x, -1, np.float64)
+    assert abs(np.sum(np.abs(X) ** 2) - h * w * np.sum(np.abs(x) ** 2)) < 1e-9 * np.sum(np.abs(X) ** 2)
+    assert np.allclose(oracle.fft2d(X, +1, np.float64), h * w * x, atol=1e-9)
+    assert np.allclose(oracle.fft2d(2 * x + 3j * y, -1, np.float64),
+                       2 * X + 3j * oracle.fft2d(y, -1, np.float64), atol=1e-9)
+    assert np.allclose(X, np.fft.fft2(x), atol=1e-10)
+
+
+# ------------------------------------------------------------ shrinkWrap
+def test_shrinkwrap_zero_cycles_is_ifft_real_part():
+    rng = np.random.default_rng(2)
+    M = rng.random((32, 48), dtype=np.float32)
+    out, st = oracle.shrink_wrap(M, n_cycles=0)
+    assert st.cycles_run == 0
+    exp = np.real(np.fft.ifft2(M.astype(np.float64))) * M.size
+    assert np.linalg.norm(out - exp) <= 1e-6 * np.linalg.norm(exp)
+
+
+def test_shrinkwrap_early_exit_and_defaults():
+    from imresh_amd import synthetic as syn
+    obj, M = syn.workload(32, 32, n_shapes=4, central=16, index=3)
+    out, st = oracle.shrink_wrap(M, n_cycles=7, target_error=1e9, n_hio_cycles=2)
+    assert st.cycles_run == 1                       # shrinkWrap.cpp:328-329
+    a, sa = oracle.shrink_wrap(M, n_cycles=2, target_error=0, hio_beta=0, cutoff_autocorel=0,
+                               cutoff=0, sigma0=0, sigma_change=0, n_hio_cycles=0)
+    b, sb = oracle.shrink_wrap(M, n_cycles=2)
+    assert np.array_equal(a, b)                     # <= 0 selects the defaults (:134-140)
+
+
+def test_shrinkwrap_fp32_tracks_fp64_on_short_runs():
+    from imresh_amd import synthetic as syn
+    obj, M = syn.workload(64, 64, n_shapes=8, central=32, index=1)
+    a, sa = oracle.shrink_wrap(M, n_cycles=1, n_hio_cycles=3, target_error=1e-38)
+    b, sb = oracle.shrink_wrap(M, n_cycles=1, n_hio_cycles=3, target_error=1e-38, dtype=np.float64)
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+    assert abs(sa.last_error - sb.last_error) <= 1e-5 * sb.last_error
+
+
+def test_make_mask_cycle_zero_is_autocorrelation_support():
+    """The cycle-0 mask (1 = outside support) thresholds the blurred |IFFT(M)|
+    at 0.04 of its maximum; the DC peak (max of the autocorrelation) is inside."""
+    from imresh_amd import synthetic as syn
+    obj, M = syn.workload(64, 64, n_shapes=8, central=32, index=2)
+    g = oracle.fft2d(M.astype(np.complex64), +1, np.float32)
+    mask = oracle.make_mask(g, 3.0, 0.04)
+    assert mask[0, 0] == 0 and set(np.unique(mask)) <= {0.0, 1.0}
+    b = oracle.blur(oracle.complex_norm(g), 3.0)
+    assert np.array_equal(mask, (b < np.float32(b.max() * np.float32(0.04))).astype(np.float32))
+
+
+# ------------------------------------------------------------ golden fixtures
+def test_golden_fixtures():
+    """tests/golden/oracle_golden.npz (make_golden.py): blur outputs of the
+    reference's own CPU sources, FFTs from numpy's pocketfft, a short
+    float64 shrink-wrap run and the KAT inputs."""
+    z = np.load(GOLDEN, allow_pickle=False)
+    for key in [k for k in z.files if k.startswith("blur_in_")]:
+        tag = key[len("blur_in_"):]
+        sigma = float(z[f"blur_sigma_{tag}"])
+        assert np.array_equal(oracle.blur(z[key], sigma, "hv"), z[f"blur_out_{tag}"]), tag
+    x = z["fft_in"]
+    assert np.allclose(oracle.fft2d(x, -1, np.float64), z["fft_out"], rtol=0, atol=1e-10)
+    for s, w in zip(z["kern_sigmas"], z["kern_weights"]):
+        n = len(oracle.gauss_kernel(s))
+        assert np.array_equal(oracle.gauss_kernel(s), w[:n])
+    out, st = oracle.shrink_wrap(z["sw_modulus"], n_cycles=2, n_hio_cycles=3,
+                                 target_error=1e-38, dtype=np.float64)
+    assert np.allclose(out, z["sw_out_f64"], rtol=1e-9, atol=1e-9)
