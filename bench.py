@@ -108,6 +108,9 @@ def main() -> int:
         im.check(L.imresh_memcpy_d2d(d_io.ptr, d_src.ptr, M.nbytes, None), "d2d")
         im.check(L.imresh_shrinkwrap_device(d_io.ptr, W, H, p, None, None), "shrinkwrap")
 
+    rv, cv = im.ctypes.c_int(), im.ctypes.c_int()
+    im.check(L.imresh_autotune(W, H, None, im.ctypes.byref(rv), im.ctypes.byref(cv)), "autotune")
+    log(f"[bench] autotuned kernel variants: row {rv.value}, column {cv.value}")
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
@@ -162,7 +165,8 @@ def main() -> int:
         "data": "synthetic (seeded rectangles/discs, Poisson 1e9 photons)",
         "config": {"workload": "shrink-wrap 2048x2048 fp32, 20 cycles x 20 HIO, beta 0.9, "
                                "sigma0 3.0, cut 0.2/0.04, 64 shapes in central 1024^2",
-                   "iterations_per_step": N_CYCLES * N_HIO, "width": W, "height": H},
+                   "iterations_per_step": N_CYCLES * N_HIO, "width": W, "height": H,
+                   "row_variant": rv.value, "col_variant": cv.value},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -20,6 +20,8 @@
 // reconstruction into a no-op, so the host never waits inside the loop.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -194,8 +196,73 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
     return IMRESH_OK;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Picks, per device and image size, the fastest row-pass and column-pass
+// kernel variant (element budget / rows or columns per workgroup / LDS
+// exchange) by timing each on a synthetic resident state with HIP events.
+// Runs once per (device, W, H) and process; IMRESH_AUTOTUNE=0 keeps the
+// defaults, IMRESH_VARIANTS=r,c forces a choice.
+int time_variant(bool row, int var, Plan2D p, Workspace *w, hipStream_t s, float *ms)
+{
+    RowArgs ra{};
+    ra.X = w->X; ra.G = w->G; ra.mask = w->mask; ra.Bn = w->Bn; ra.B2 = w->B2;
+    ra.thr = w->scal; ra.beta = 0.9f; ra.perr = w->perr; ra.pcnt = w->pcnt;
+    ColArgs ca{};
+    ca.X = w->X; ca.M = w->M;
+    if (row) p.rowVar = var; else p.colVar = var;
+    IMR_CHECK(launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, w->W, s));
+    hipEvent_t e0, e1;
+    IMR_HIP_CHECK(hipEventCreate(&e0));
+    IMR_HIP_CHECK(hipEventCreate(&e1));
+    int st = IMRESH_OK;
+    constexpr int reps = 4;
+    for (int r = 0; r <= reps && st == IMRESH_OK; ++r) {
+        if (r == 1) hipEventRecord(e0, s);   // first launch warms up
+        st = row ? launch_row(ROW_MID, p, ra, s) : launch_col(COL_PROJ, p, ca, s);
+    }
+    hipEventRecord(e1, s);
+    if (st == IMRESH_OK) st = from_hip(hipEventSynchronize(e1));
+    float t = 0.f;
+    if (st == IMRESH_OK) hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = t / reps;
+    return st;
+}
+
+int autotune(int device, int W, int H, hipStream_t s, Workspace *w)
+{
+    int rv = 0, cv = 0;
+    if (get_tuned_variants(device, W, H, &rv, &cv)) return IMRESH_OK;
+    Plan2D p;
+    IMR_CHECK(get_plan2d(device, W, H, &p));
+    const int nr = row_variants(p.row), nc = col_variants(p.col);
+    const char *force = std::getenv("IMRESH_VARIANTS");
+    const char *off = std::getenv("IMRESH_AUTOTUNE");
+    if (force != nullptr && std::sscanf(force, "%d,%d", &rv, &cv) == 2) {
+        rv = std::min(std::max(rv, 0), std::max(nr - 1, 0));
+        cv = std::min(std::max(cv, 0), std::max(nc - 1, 0));
+    } else if (!(off != nullptr && off[0] == '0') && (nr > 1 || nc > 1)) {
+        float best = 1e30f;
+        for (int v = 0; v < nr && nr > 1; ++v) {
+            float ms;
+            IMR_CHECK(time_variant(true, v, p, w, s, &ms));
+            if (ms < best) { best = ms; rv = v; }
+        }
+        best = 1e30f;
+        for (int v = 0; v < nc && nc > 1; ++v) {
+            float ms;
+            IMR_CHECK(time_variant(false, v, p, w, s, &ms));
+            if (ms < best) { best = ms; cv = v; }
+        }
+    }
+    set_tuned_variants(device, W, H, rv, cv);
+    return IMRESH_OK;
+}
+
 int run_reconstruction(Workspace *w, float *io_dev, const Params &q, hipStream_t s)
 {
+    IMR_CHECK(autotune(w->device, w->W, w->H, s, w));
     Plan2D p;
     IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
     IMR_CHECK(ensure_weights(w, q, s));
@@ -504,9 +571,21 @@ int imresh_hio_iterations(float *gprime_dev, float *gprev_dev, const uint8_t *ma
     if (n_iter == 0) return IMRESH_OK;
     int dev;
     IMR_CHECK(current_device(&dev));
+    hipStream_t s = S(stream);
+    {
+        int rv, cv;
+        if (!get_tuned_variants(dev, width, height, &rv, &cv)) {
+            Plan2D p0;
+            IMR_CHECK(get_plan2d(dev, width, height, &p0));
+            Workspace *w;
+            IMR_CHECK(ws_checkout(dev, width, height, s, &w));
+            const int st = autotune(dev, width, height, s, w);
+            ws_return(w, s);
+            IMR_CHECK(st);
+        }
+    }
     Plan2D p;
     IMR_CHECK(get_plan2d(dev, width, height, &p));
-    hipStream_t s = S(stream);
     RowArgs ra{};
     ra.X = (float2 *) gprime_dev;
     ra.G = (float2 *) gprev_dev;
@@ -620,62 +699,85 @@ int imresh_fft2d(float *out, const float *in, uint32_t W, uint32_t H, int direct
 
 int imresh_fft_supported(uint32_t n) { return line_supported((int) n) ? 1 : 0; }
 
+int imresh_autotune(uint32_t width, uint32_t height, void *stream, int *row_variant,
+                    int *col_variant)
+{
+    if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, width, height, &p));
+    hipStream_t s = S(stream);
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, width, height, s, &w));
+    int st = autotune(dev, width, height, s, w);
+    ws_return(w, s);
+    IMR_CHECK(st);
+    IMR_CHECK(get_plan2d(dev, width, height, &p));
+    if (row_variant) *row_variant = p.rowVar;
+    if (col_variant) *col_variant = p.colVar;
+    return IMRESH_OK;
+}
+
 int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms, void *stream)
 {
     if (out_ms == nullptr) return IMRESH_ERR_NULL;
     if (W == 0 || H == 0 || reps == 0) return IMRESH_ERR_INVALID;
     int dev;
     IMR_CHECK(current_device(&dev));
-    Plan2D p;
-    IMR_CHECK(get_plan2d(dev, W, H, &p));
     hipStream_t s = S(stream);
     Workspace *w;
     IMR_CHECK(ws_checkout(dev, W, H, s, &w));
-    int st = IMRESH_OK;
     Params q;
     normalize(nullptr, q);
     q.n_cycles = 1;
-    hipEvent_t ev[5];
-    for (auto &e : ev) hipEventCreate(&e);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     double acc[4] = {0, 0, 0, 0};
-    do {
-        if ((st = ensure_weights(w, q, s)) != IMRESH_OK) break;
-        if ((st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, W, s)) != IMRESH_OK) break;
-        if ((st = from_hip(hipMemsetAsync(w->flags, 0, 64, s))) != IMRESH_OK) break;
-        RowArgs ra{};
-        ra.X = w->X; ra.G = w->G; ra.mask = w->mask; ra.Bn = w->Bn; ra.B2 = w->B2;
-        ra.thr = w->scal; ra.beta = 0.9f; ra.perr = w->perr; ra.pcnt = w->pcnt;
-        ColArgs ca{};
-        ca.X = w->X; ca.M = w->M;
-        BlurArgs ba{};
-        ba.in = w->Bn; ba.out = w->B2; ba.wh = ba.wv = w->weights;
-        ba.rh = ba.rv = (w->taps[0] - 1) / 2; ba.W = W; ba.H = H; ba.pmax = w->pmax;
-        // one untimed round to warm caches and code objects
-        for (uint32_t r = 0; r <= reps && st == IMRESH_OK; ++r) {
-            hipEventRecord(ev[0], s);
-            st = launch_row(ROW_MID, p, ra, s);
-            hipEventRecord(ev[1], s);
-            if (st == IMRESH_OK) st = launch_col(COL_PROJ, p, ca, s);
-            hipEventRecord(ev[2], s);
-            if (st == IMRESH_OK) st = launch_blur(ba, s);
-            hipEventRecord(ev[3], s);
-            if (st == IMRESH_OK) st = launch_row(ROW_LAST, p, ra, s);
-            hipEventRecord(ev[4], s);
+    int st = autotune(dev, W, H, s, w);
+    Plan2D p;
+    if (st == IMRESH_OK) st = get_plan2d(dev, W, H, &p);
+    if (st == IMRESH_OK) st = ensure_weights(w, q, s);
+    if (st == IMRESH_OK) st = from_hip(hipEventCreate(&e0));
+    if (st == IMRESH_OK) st = from_hip(hipEventCreate(&e1));
+    RowArgs ra{};
+    ra.X = w->X; ra.G = w->G; ra.mask = w->mask; ra.Bn = w->Bn; ra.B2 = w->B2;
+    ra.thr = w->scal; ra.beta = 0.9f; ra.perr = w->perr; ra.pcnt = w->pcnt;
+    ColArgs ca{};
+    ca.X = w->X; ca.M = w->M;
+    BlurArgs ba{};
+    ba.in = w->Bn; ba.out = w->B2; ba.wh = ba.wv = w->weights;
+    ba.W = W; ba.H = H; ba.pmax = w->pmax;
+    if (st == IMRESH_OK) ba.rh = ba.rv = (w->taps[0] - 1) / 2;
+    // each kernel back to back in batches of 4 launches on a freshly filled
+    // state (unnormalized transforms grow the values by n per launch)
+    constexpr int batch = 4;
+    const int nb = (int) ((reps + batch - 1) / batch);
+    for (int k = 0; k < 4 && st == IMRESH_OK; ++k) {
+        for (int b = -1; b < nb && st == IMRESH_OK; ++b) {   // batch -1 warms up
+            st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, W, s);
             if (st != IMRESH_OK) break;
-            if ((st = from_hip(hipEventSynchronize(ev[4]))) != IMRESH_OK) break;
-            if (r == 0) continue;
-            for (int k = 0; k < 4; ++k) {
-                float ms = 0;
-                hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-                acc[k] += ms;
+            hipEventRecord(e0, s);
+            for (int r = 0; r < batch && st == IMRESH_OK; ++r) {
+                switch (k) {
+                case 0: st = launch_row(ROW_MID, p, ra, s); break;
+                case 1: st = launch_col(COL_PROJ, p, ca, s); break;
+                case 2: st = launch_blur(ba, s); break;
+                default: st = launch_row(ROW_LAST, p, ra, s); break;
+                }
             }
+            hipEventRecord(e1, s);
+            if (st == IMRESH_OK) st = from_hip(hipEventSynchronize(e1));
+            float ms = 0.f;
+            if (st == IMRESH_OK) hipEventElapsedTime(&ms, e0, e1);
+            if (b >= 0) acc[k] += ms;
         }
-    } while (false);
-    for (auto &e : ev) hipEventDestroy(e);
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
     hipStreamSynchronize(s);
     ws_return(w, s);
     if (st == IMRESH_OK)
-        for (int k = 0; k < 4; ++k) out_ms[k] = (float) (acc[k] / reps);
+        for (int k = 0; k < 4; ++k) out_ms[k] = (float) (acc[k] / ((double) nb * batch));
     return st;
 }
 

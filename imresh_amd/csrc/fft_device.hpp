@@ -183,139 +183,181 @@ __device__ __forceinline__ void bfly(cf *v)
 }
 
 // ------------------------------------------------------------ LDS layout
-// One padding slot every 32 complex values spreads the stride-R scatters of
-// the early passes over the LDS banks.
+// One padding slot every 32 values spreads the stride-R scatters of the
+// early passes over the LDS banks.
 __host__ __device__ constexpr int lpad(int i) { return i + (i >> 5); }
 __host__ __device__ constexpr int lds_elems(int n) { return n + (n >> 5) + 1; }
 
-struct LdsLine {
+// One line in LDS: element i at p[lpad(base + i*stride)].  LdsC holds
+// complex values; LdsS holds one float component at a time (the exchange
+// then runs once for the real and once for the imaginary parts, halving the
+// LDS footprint of a tile).
+struct LdsC {
     cf *p;
-    int base;
-    int stride;
+    int base, stride;
     __device__ __forceinline__ cf &operator()(int i) const { return p[lpad(base + i * stride)]; }
+};
+struct LdsS {
+    float *p;
+    int base, stride;
+    __device__ __forceinline__ float &operator()(int i) const { return p[lpad(base + i * stride)]; }
 };
 
 // ------------------------------------------------------------ passes
-// Pass I of plan P in direction DIR (sequence order: forward = plan order,
-// inverse = reversed).  SRC: gather from `src` (global) instead of LDS;
-// DST: scatter to `dst` (global) instead of LDS.
-template <class P, int DIR, int I, bool SRC, bool DST, class Src, class Dst>
-__device__ __forceinline__ void cpass(cf (&v)[P::E], const cf *__restrict__ tw, int t,
-                                      const LdsLine &L, Src &src, Dst &dst)
-{
-    constexpr bool INV = DIR > 0;
-    constexpr int R = P::rad(INV, I), NS = P::ns(INV, I), TWOFF = P::twoff(INV, I);
-    constexpr int T = P::T, nb = P::n / R, B = (nb + T - 1) / T;
+// Pass I of plan P in direction DIR (forward: plan order, inverse: reversed).
+template <class P, int DIR, int I>
+struct PassOf {
+    static constexpr bool INV = DIR > 0;
+    static constexpr int R = P::rad(INV, I), NS = P::ns(INV, I), TWOFF = P::twoff(INV, I);
+    static constexpr int T = P::T, nb = P::n / R, B = (nb + T - 1) / T;
     static_assert(B * R <= P::E, "plan exceeds the per-thread element budget");
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = t + b * T;
-        if (nb % T == 0 || j < nb) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if constexpr (SRC) v[b * R + r] = src(j + r * nb);
-                else v[b * R + r] = L(j + r * nb);
-            }
-        }
-    }
-    if constexpr (!SRC) __syncthreads();
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = t + b * T;
-        if (nb % T == 0 || j < nb) {
-            const int k = j % NS;
-            if constexpr (NS > 1) {
-                const cf *w = tw + TWOFF + k * (R - 1);
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[b * R + r] = twmul<DIR>(v[b * R + r], w[r - 1]);
-            }
-            bfly<R, DIR>(&v[b * R]);
-            const int base = (j - k) * R + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if constexpr (DST) dst(base + r * NS, v[b * R + r]);
-                else L(base + r * NS) = v[b * R + r];
-            }
-        }
-    }
-    if constexpr (!DST) __syncthreads();
-}
+    __device__ static constexpr bool full() { return nb % T == 0; }
+};
 
-// passes [I, I1) of direction DIR; pass 0 gathers from src, pass np-1
-// scatters to dst, everything else goes through LDS
-template <class P, int DIR, int I, int I1, class Src, class Dst>
-__device__ __forceinline__ void cpasses(cf (&v)[P::E], const cf *tw, int t, const LdsLine &L,
-                                        Src &src, Dst &dst)
+// gather the inputs of pass I (natural positions j + r*nb) from global memory
+template <class P, int DIR, int I, class Src>
+__device__ __forceinline__ void gather_global(cf (&v)[P::E], int t, Src &src)
 {
-    if constexpr (I < I1) {
-        cpass<P, DIR, I, I == 0, I == P::np - 1>(v, tw, t, L, src, dst);
-        cpasses<P, DIR, I + 1, I1>(v, tw, t, L, src, dst);
+    using Q = PassOf<P, DIR, I>;
+#pragma unroll
+    for (int b = 0; b < Q::B; ++b) {
+        const int j = t + b * Q::T;
+        if (Q::full() || j < Q::nb) {
+#pragma unroll
+            for (int r = 0; r < Q::R; ++r) v[b * Q::R + r] = src(j + r * Q::nb);
+        }
     }
 }
 
-// One full transform: global (src) -> ... -> global (dst).
-template <class P, int DIR, class Src, class Dst>
-__device__ __forceinline__ void line_fft(cf (&v)[P::E], const cf *tw, int t, const LdsLine &L,
-                                         Src &src, Dst &dst)
+// twiddle multiply + radix-R butterflies of pass I, in registers
+template <class P, int DIR, int I>
+__device__ __forceinline__ void compute(cf (&v)[P::E], const cf *__restrict__ tw, int t)
 {
-    cpasses<P, DIR, 0, P::np>(v, tw, t, L, src, dst);
+    using Q = PassOf<P, DIR, I>;
+#pragma unroll
+    for (int b = 0; b < Q::B; ++b) {
+        const int j = t + b * Q::T;
+        if (Q::full() || j < Q::nb) {
+            if constexpr (Q::NS > 1) {
+                const int k = j % Q::NS;
+                const cf *w = tw + Q::TWOFF + k * (Q::R - 1);
+#pragma unroll
+                for (int r = 1; r < Q::R; ++r) v[b * Q::R + r] = twmul<DIR>(v[b * Q::R + r], w[r - 1]);
+            }
+            bfly<Q::R, DIR>(&v[b * Q::R]);
+        }
+    }
+}
+
+// output positions of pass I: (j - j%ns)*R + j%ns + r*ns
+template <class P, int DIR, int I, class F>
+__device__ __forceinline__ void for_outputs(int t, F &&f)
+{
+    using Q = PassOf<P, DIR, I>;
+#pragma unroll
+    for (int b = 0; b < Q::B; ++b) {
+        const int j = t + b * Q::T;
+        if (Q::full() || j < Q::nb) {
+            const int k = j % Q::NS;
+            const int base = (j - k) * Q::R + k;
+#pragma unroll
+            for (int r = 0; r < Q::R; ++r) f(b * Q::R + r, base + r * Q::NS);
+        }
+    }
+}
+
+// input positions of pass I: j + r*nb
+template <class P, int DIR, int I, class F>
+__device__ __forceinline__ void for_inputs(int t, F &&f)
+{
+    using Q = PassOf<P, DIR, I>;
+#pragma unroll
+    for (int b = 0; b < Q::B; ++b) {
+        const int j = t + b * Q::T;
+        if (Q::full() || j < Q::nb) {
+#pragma unroll
+            for (int r = 0; r < Q::R; ++r) f(b * Q::R + r, j + r * Q::nb);
+        }
+    }
+}
+
+// scatter the outputs of the last pass to global memory (natural order)
+template <class P, int DIR, int I, class Dst>
+__device__ __forceinline__ void scatter_global(cf (&v)[P::E], int t, Dst &dst)
+{
+    for_outputs<P, DIR, I>(t, [&](int s, int e) { dst(e, v[s]); });
+}
+
+// elementwise operator on the natural-order outputs of a last pass
+template <class P, int DIR, int I, class Mid>
+__device__ __forceinline__ void apply_mid(cf (&v)[P::E], int t, Mid &mid)
+{
+    for_outputs<P, DIR, I>(t, [&](int s, int e) { v[s] = mid(e, v[s]); });
+}
+
+// LDS exchange between pass IA of direction DA (its outputs) and pass IB of
+// direction DB (its inputs): write, barrier, read, barrier.
+template <class P, int DA, int IA, int DB, int IB>
+__device__ __forceinline__ void exchange(cf (&v)[P::E], int t, const LdsC &L)
+{
+    for_outputs<P, DA, IA>(t, [&](int s, int e) { L(e) = v[s]; });
+    __syncthreads();
+    for_inputs<P, DB, IB>(t, [&](int s, int e) { v[s] = L(e); });
+    __syncthreads();
+}
+template <class P, int DA, int IA, int DB, int IB>
+__device__ __forceinline__ void exchange(cf (&v)[P::E], int t, const LdsS &L)
+{
+    for_outputs<P, DA, IA>(t, [&](int s, int e) { L(e) = v[s].x; });
+    __syncthreads();
+    for_inputs<P, DB, IB>(t, [&](int s, int e) { v[s].x = L(e); });
+    __syncthreads();
+    for_outputs<P, DA, IA>(t, [&](int s, int e) { L(e) = v[s].y; });
+    __syncthreads();
+    for_inputs<P, DB, IB>(t, [&](int s, int e) { v[s].y = L(e); });
+    __syncthreads();
+}
+
+// passes [I, np) of direction DIR, each preceded by an LDS exchange
+template <class P, int DIR, int I, class Lds>
+__device__ __forceinline__ void passes_from(cf (&v)[P::E], const cf *tw, int t, const Lds &L)
+{
+    if constexpr (I < P::np) {
+        exchange<P, DIR, I - 1, DIR, I>(v, t, L);
+        compute<P, DIR, I>(v, tw, t);
+        passes_from<P, DIR, I + 1>(v, tw, t, L);
+    }
+}
+
+// One full transform: global (src) -> registers/LDS -> global (dst).
+template <class P, int DIR, class Lds, class Src, class Dst>
+__device__ __forceinline__ void line_fft(cf (&v)[P::E], const cf *tw, int t, const Lds &L, Src &src,
+                                         Dst &dst)
+{
+    gather_global<P, DIR, 0>(v, t, src);
+    compute<P, DIR, 0>(v, tw, t);
+    passes_from<P, DIR, 1>(v, tw, t, L);
+    scatter_global<P, DIR, P::np - 1>(v, t, dst);
 }
 
 // Transform in direction DA, elementwise `mid` on its natural-order output,
-// then the transform in direction DB -- the junction stays in registers.
-template <class P, int DA, int DB, class Src, class Mid, class Dst>
-__device__ __forceinline__ void line_pair(cf (&v)[P::E], const cf *tw, int t, const LdsLine &L,
-                                          Src &src, Mid &mid, Dst &dst)
+// then the transform in direction DB.  A's last pass and B's first pass
+// have the same radix (mirrored sequences), so the junction never leaves
+// registers: B's first pass consumes exactly the positions A's last pass
+// produced in the same thread.
+template <class P, int DA, int DB, class Lds, class Src, class Mid, class Dst>
+__device__ __forceinline__ void line_pair(cf (&v)[P::E], const cf *tw, int t, const Lds &L, Src &src,
+                                          Mid &mid, Dst &dst)
 {
     static_assert(DA == -DB, "a pair runs opposite directions");
-    constexpr int np = P::np, T = P::T;
-    constexpr bool INVA = DA > 0;
-    // all passes of A but the last
-    cpasses<P, DA, 0, np - 1>(v, tw, t, L, src, dst);
-    // last pass of A (radix R) + mid + first pass of B (same radix, span 1)
-    constexpr int I = np - 1;
-    constexpr int R = P::rad(INVA, I), NS = P::ns(INVA, I), TWOFF = P::twoff(INVA, I);
-    static_assert(R == P::rad(!INVA, 0), "mirrored sequences share the junction radix");
-    constexpr int nb = P::n / R, B = (nb + T - 1) / T;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = t + b * T;
-        if (nb % T == 0 || j < nb) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if constexpr (np == 1) v[b * R + r] = src(j + r * nb);
-                else v[b * R + r] = L(j + r * nb);
-            }
-        }
-    }
-    if constexpr (np > 1) __syncthreads();
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = t + b * T;
-        if (nb % T == 0 || j < nb) {
-            if constexpr (NS > 1) {
-                const int k = j % NS;
-                const cf *w = tw + TWOFF + k * (R - 1);
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[b * R + r] = twmul<DA>(v[b * R + r], w[r - 1]);
-            }
-            bfly<R, DA>(&v[b * R]);
-            // the last pass writes natural order j + r*nb (j < nb == NS)
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[b * R + r] = mid(j + r * nb, v[b * R + r]);
-            bfly<R, DB>(&v[b * R]);
-            // first pass of B: span 1, output j*R + r
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if constexpr (np == 1) dst(j * R + r, v[b * R + r]);
-                else L(j * R + r) = v[b * R + r];
-            }
-        }
-    }
-    if constexpr (np > 1) {
-        __syncthreads();
-        cpasses<P, DB, 1, np>(v, tw, t, L, src, dst);
-    }
+    static_assert(P::rad(DA > 0, P::np - 1) == P::rad(DB > 0, 0), "mirrored junction radix");
+    gather_global<P, DA, 0>(v, t, src);
+    compute<P, DA, 0>(v, tw, t);
+    passes_from<P, DA, 1>(v, tw, t, L);
+    apply_mid<P, DA, P::np - 1>(v, t, mid);
+    compute<P, DB, 0>(v, tw, t);
+    passes_from<P, DB, 1>(v, tw, t, L);
+    scatter_global<P, DB, P::np - 1>(v, t, dst);
 }
 
 // ------------------------------------------------------------ reductions

@@ -86,7 +86,14 @@ struct Plan2D {
     int W, H;
     LinePlan row;    // n = W
     LinePlan col;    // n = H
+    int rowVar;      // kernel variant of the row pass (autotuned)
+    int colVar;      // kernel variant of the column pass (autotuned)
 };
+
+int row_variants(const LinePlan &lp);
+int col_variants(const LinePlan &lp);
+void set_tuned_variants(int device, int W, int H, int rowVar, int colVar);
+bool get_tuned_variants(int device, int W, int H, int *rowVar, int *colVar);
 
 int get_line_plan(int device, int n, LinePlan *out);
 int get_plan2d(int device, int W, int H, Plan2D *out);
@@ -95,14 +102,18 @@ bool line_supported(int n);
 int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s);
 int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s);
 // per translation-unit group (k_row.hip / k_col.hip compiled once per group)
-#define IMR_DECLARE_GROUP(g)                                                         \
-    int launch_row_g##g(int mode, int n, const LinePlan &lp, RowArgs a, hipStream_t s); \
-    int launch_col_g##g(int mode, int n, const LinePlan &lp, ColArgs a, hipStream_t s);
+#define IMR_DECLARE_GROUP(g)                                                                  \
+    int launch_row_g##g(int mode, int var, int n, const LinePlan &lp, RowArgs a, hipStream_t s); \
+    int launch_col_g##g(int mode, int var, int n, const LinePlan &lp, ColArgs a, hipStream_t s); \
+    int row_variants_g##g(int n);                                                            \
+    int row_rows_g##g(int n, int var);                                                       \
+    int col_variants_g##g(int n);
 IMR_DECLARE_GROUP(0)
 IMR_DECLARE_GROUP(1)
 IMR_DECLARE_GROUP(2)
 IMR_DECLARE_GROUP(3)
 IMR_DECLARE_GROUP(4)
+// number of row workgroups of the plan's row variant
 int row_grid(const Plan2D &p);
 int blur_grid(int W, int H);
 int launch_blur(const BlurArgs &a, hipStream_t s);

@@ -14,6 +14,8 @@
 #include "ops_device.hpp"
 #include "plans.hpp"
 
+#include <type_traits>
+
 #ifndef IMR_GROUP
 #error "compile with -DIMR_GROUP=<plan group>"
 #endif
@@ -24,12 +26,13 @@ namespace {
 
 enum ColKernel { CK_FWD, CK_INV, CK_PROJ };
 
-template <class P, int K>
-__global__ void __launch_bounds__(P::C *P::T) k_col(ColArgs a)
+// C columns per workgroup; SPLIT: exchange real and imaginary parts through
+// LDS one after the other (half the LDS of a complex tile).
+template <class P, int C, bool SPLIT, int K>
+__global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 {
     if (a.done != nullptr && *a.done) return;
     extern __shared__ cf smem[];
-    constexpr int C = P::C;
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int col = blockIdx.x * C + c;
     const bool valid = col < a.W;
@@ -38,7 +41,8 @@ __global__ void __launch_bounds__(P::C *P::T) k_col(ColArgs a)
     // the edge read column 0 of the tile and never store
     const size_t wb = (size_t) blockIdx.x * C;
     const int co = valid ? c : 0;
-    const LdsLine L{smem, c, C};
+    using Lds = std::conditional_t<SPLIT, LdsS, LdsC>;
+    const Lds L{(decltype(Lds::p)) smem, c, C};
     const cf *tw = (const cf *) a.tw;
     cf v[P::E];
 
@@ -66,16 +70,23 @@ __global__ void __launch_bounds__(P::C *P::T) k_col(ColArgs a)
     }
 }
 
-template <class P>
-int launch_plan(int mode, ColArgs a, hipStream_t s)
+// LDS bytes of a C-column tile
+template <class P, int C, bool SPLIT>
+constexpr size_t col_lds() { return (SPLIT ? sizeof(float) : sizeof(cf)) * (size_t) lds_elems(C * P::n); }
+
+template <class P, int C, bool SPLIT>
+int launch_variant(int mode, ColArgs a, hipStream_t s)
 {
-    const dim3 grid((a.W + P::C - 1) / P::C), block(P::C * P::T);
-    const size_t lds = sizeof(cf) * (size_t) lds_elems(P::C * P::n);
-    if (lds > 64 * 1024) {
+    static_assert(C * P::T <= 1024, "column workgroup too large");
+    static_assert(col_lds<P, C, SPLIT>() <= kLdsBudget, "column tile exceeds the LDS budget");
+    const dim3 grid((a.W + C - 1) / C), block(C * P::T);
+    constexpr size_t lds = col_lds<P, C, SPLIT>();
+    if constexpr (lds > 64 * 1024) {
         static const hipError_t attr = [&] {
             hipError_t e = hipSuccess;
-            for (const void *f : {(const void *) k_col<P, CK_FWD>, (const void *) k_col<P, CK_INV>,
-                                  (const void *) k_col<P, CK_PROJ>})
+            for (const void *f : {(const void *) k_col<P, C, SPLIT, CK_FWD>,
+                                  (const void *) k_col<P, C, SPLIT, CK_INV>,
+                                  (const void *) k_col<P, C, SPLIT, CK_PROJ>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -83,25 +94,69 @@ int launch_plan(int mode, ColArgs a, hipStream_t s)
         if (attr != hipSuccess) return from_hip(attr);
     }
     switch (mode) {
-    case COL_FWD: a.flags = 0; hipLaunchKernelGGL((k_col<P, CK_FWD>), grid, block, lds, s, a); break;
-    case COL_INV: a.flags = 0; hipLaunchKernelGGL((k_col<P, CK_INV>), grid, block, lds, s, a); break;
+    case COL_FWD:
+        a.flags = 0;
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_FWD>), grid, block, lds, s, a);
+        break;
+    case COL_INV:
+        a.flags = 0;
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV>), grid, block, lds, s, a);
+        break;
     case COL_INIT:
         a.flags = kColRealIn;
         a.in = a.X;   // unused with kColRealIn; never a null pointer in a kernel
         a.out = a.X;
-        hipLaunchKernelGGL((k_col<P, CK_INV>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV>), grid, block, lds, s, a);
         break;
-    case COL_PROJ: a.flags = 0; hipLaunchKernelGGL((k_col<P, CK_PROJ>), grid, block, lds, s, a); break;
+    case COL_PROJ:
+        a.flags = 0;
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ>), grid, block, lds, s, a);
+        break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());
 }
 
+// Column variants of a length: 0 = E 16 with the plan's default tile; for
+// long columns also E 32 (half the threads per column, 512-thread groups)
+// and E 32 with 16-column split-LDS tiles (128-byte row segments).
+template <int N, int... Rs>
+int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
+{
+    using P16 = PlanE<16, N, Rs...>;
+    if constexpr (N >= 1024) {
+        using P32 = PlanE<32, N, Rs...>;
+        if (var == 1) return launch_variant<P32, P32::C, false>(mode, a, s);
+        if constexpr (16 * P32::T <= 1024 && col_lds<P32, 16, true>() <= kLdsBudget)
+            if (var == 2) return launch_variant<P32, 16, true>(mode, a, s);
+    }
+    return launch_variant<P16, P16::C, false>(mode, a, s);
+}
+
+template <int N, int... Rs>
+constexpr int col_variant_count()
+{
+    if constexpr (N >= 1024) {
+        using P32 = PlanE<32, N, Rs...>;
+        return (16 * P32::T <= 1024 && col_lds<P32, 16, true>() <= kLdsBudget) ? 3 : 2;
+    }
+    return 1;
+}
+
 template <int G, int N, int... Rs>
-int try_plan(int mode, int n, ColArgs a, hipStream_t s)
+int try_plan(int mode, int var, int n, ColArgs a, hipStream_t s)
 {
     if constexpr (G == IMR_GROUP) {
-        if (n == N) return launch_plan<Plan<N, Rs...>>(mode, a, s);
+        if (n == N) return launch_col_variant<N, Rs...>(var, mode, a, s);
+    }
+    return -1;
+}
+
+template <int G, int N, int... Rs>
+int try_count(int n)
+{
+    if constexpr (G == IMR_GROUP) {
+        if (n == N) return col_variant_count<N, Rs...>();
     }
     return -1;
 }
@@ -112,15 +167,27 @@ int try_plan(int mode, int n, ColArgs a, hipStream_t s)
 #define IMR_CAT(a, b) IMR_CAT2(a, b)
 #define IMR_COL_CASE(g, N, ...)                                                     \
     {                                                                               \
-        const int r = try_plan<g, N, __VA_ARGS__>(mode, n, a, s);                   \
+        const int r = try_plan<g, N, __VA_ARGS__>(mode, var, n, a, s);              \
+        if (r >= 0) return r;                                                       \
+    }
+#define IMR_COL_COUNT(g, N, ...)                                                    \
+    {                                                                               \
+        const int r = try_count<g, N, __VA_ARGS__>(n);                              \
         if (r >= 0) return r;                                                       \
     }
 
-int IMR_CAT(launch_col_g, IMR_GROUP)(int mode, int n, const LinePlan &lp, ColArgs a, hipStream_t s)
+int IMR_CAT(launch_col_g, IMR_GROUP)(int mode, int var, int n, const LinePlan &lp, ColArgs a,
+                                     hipStream_t s)
 {
     a.tw = lp.tw;
     IMR_PLANS(IMR_COL_CASE)
     return IMRESH_ERR_UNSUPPORTED_SIZE;
+}
+
+int IMR_CAT(col_variants_g, IMR_GROUP)(int n)
+{
+    IMR_PLANS(IMR_COL_COUNT)
+    return 0;
 }
 
 }  // namespace imr

@@ -28,20 +28,21 @@ namespace {
 
 enum RowKernel { RK_FWD, RK_INV, RK_INVOUT, RK_FIRST, RK_MID };
 
-template <class P, int K>
-__global__ void __launch_bounds__(P::L *P::T) k_row(RowArgs a)
+// NL rows per workgroup
+template <class P, int NL, int K>
+__global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
 {
     if (a.done != nullptr && *a.done) return;
     extern __shared__ cf smem[];
     constexpr int n = P::n, T = P::T;
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
-    const int row = blockIdx.x * P::L + l;
+    const int row = blockIdx.x * NL + l;
     const bool valid = row < a.H;
     // workgroup base (uniform) + 32-bit per-lane offsets; rows past the end
     // read row 0 of the workgroup and never store
-    const size_t wb = (size_t) blockIdx.x * P::L * n;
+    const size_t wb = (size_t) blockIdx.x * NL * n;
     const int lo = valid ? l * n : 0;
-    const LdsLine L{smem, l * n, 1};
+    const LdsC L{smem, l * n, 1};
     const cf *tw = (const cf *) a.tw;
     cf v[P::E];
     cf *X = (cf *) a.X + wb;
@@ -56,8 +57,8 @@ __global__ void __launch_bounds__(P::L *P::T) k_row(RowArgs a)
         if constexpr (K == RK_FWD) line_fft<P, -1>(v, tw, t, L, g0, sl);
         else line_fft<P, +1>(v, tw, t, L, g0, sl);
     } else if constexpr (K == RK_INVOUT) {
-        __shared__ float red_f[P::L * P::T];
-        __shared__ unsigned red_u[P::L * P::T];
+        __shared__ float red_f[NL * P::T];
+        __shared__ unsigned red_u[NL * P::T];
         const int fl = a.flags;
         cf *G = (cf *) a.G + wb;
         float *Bn = a.Bn + wb;
@@ -120,17 +121,19 @@ __global__ void __launch_bounds__(P::L *P::T) k_row(RowArgs a)
     }
 }
 
-template <class P>
-int launch_plan(int mode, RowArgs a, hipStream_t s)
+template <class P, int NL>
+int launch_variant(int mode, RowArgs a, hipStream_t s)
 {
-    const dim3 grid((a.H + P::L - 1) / P::L), block(P::L * P::T);
-    const size_t lds = sizeof(cf) * (size_t) lds_elems(P::L * P::n);
-    if (lds > 64 * 1024) {
+    static_assert(NL * P::T <= 1024, "row workgroup too large");
+    const dim3 grid((a.H + NL - 1) / NL), block(NL * P::T);
+    constexpr size_t lds = sizeof(cf) * (size_t) lds_elems(NL * P::n);
+    static_assert(lds <= kLdsBudget, "row tile exceeds the LDS budget");
+    if constexpr (lds > 64 * 1024) {
         static const hipError_t attr = [&] {
             hipError_t e = hipSuccess;
-            for (const void *f : {(const void *) k_row<P, RK_FWD>, (const void *) k_row<P, RK_INV>,
-                                  (const void *) k_row<P, RK_INVOUT>, (const void *) k_row<P, RK_FIRST>,
-                                  (const void *) k_row<P, RK_MID>})
+            for (const void *f : {(const void *) k_row<P, NL, RK_FWD>, (const void *) k_row<P, NL, RK_INV>,
+                                  (const void *) k_row<P, NL, RK_INVOUT>,
+                                  (const void *) k_row<P, NL, RK_FIRST>, (const void *) k_row<P, NL, RK_MID>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -138,39 +141,99 @@ int launch_plan(int mode, RowArgs a, hipStream_t s)
         if (attr != hipSuccess) return from_hip(attr);
     }
     switch (mode) {
-    case ROW_FWD: a.flags = 0; hipLaunchKernelGGL((k_row<P, RK_FWD>), grid, block, lds, s, a); break;
-    case ROW_INV: a.flags = 0; hipLaunchKernelGGL((k_row<P, RK_INV>), grid, block, lds, s, a); break;
+    case ROW_FWD: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_FWD>), grid, block, lds, s, a); break;
+    case ROW_INV: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_INV>), grid, block, lds, s, a); break;
     case ROW_INIT:
         a.flags = kRowWriteG | kRowWriteBn;
-        hipLaunchKernelGGL((k_row<P, RK_INVOUT>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT>), grid, block, lds, s, a);
         break;
     case ROW_LAST:
         a.flags = kRowWriteBn | kRowError;
-        hipLaunchKernelGGL((k_row<P, RK_INVOUT>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT>), grid, block, lds, s, a);
         break;
     case ROW_LAST_PLAIN:
         a.flags = 0;
-        hipLaunchKernelGGL((k_row<P, RK_INVOUT>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT>), grid, block, lds, s, a);
         break;
     case ROW_FIRST_CUT:
         a.flags = kRowCut;
-        hipLaunchKernelGGL((k_row<P, RK_FIRST>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST>), grid, block, lds, s, a);
         break;
     case ROW_FIRST_MASK:
         a.flags = 0;
-        hipLaunchKernelGGL((k_row<P, RK_FIRST>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST>), grid, block, lds, s, a);
         break;
-    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, RK_MID>), grid, block, lds, s, a); break;
+    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID>), grid, block, lds, s, a); break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());
 }
 
+// Row variants of a length: 0 = E 16 with ~256-thread groups; for long rows
+// also E 32 (half the threads per row) and E 16 with twice the rows.
+template <int N, int... Rs>
+int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
+{
+    using P16 = PlanE<16, N, Rs...>;
+    if constexpr (N >= 1024) {
+        using P32 = PlanE<32, N, Rs...>;
+        if (var == 1) return launch_variant<P32, P32::L>(mode, a, s);
+        if constexpr (2 * P16::L * P16::T <= 1024 &&
+                      sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget)
+            if (var == 2) return launch_variant<P16, 2 * P16::L>(mode, a, s);
+    }
+    return launch_variant<P16, P16::L>(mode, a, s);
+}
+
+// rows per workgroup of a variant (the host needs it for the partial sums)
+template <int N, int... Rs>
+int row_variant_rows(int var)
+{
+    using P16 = PlanE<16, N, Rs...>;
+    if constexpr (N >= 1024) {
+        using P32 = PlanE<32, N, Rs...>;
+        if (var == 1) return P32::L;
+        if constexpr (2 * P16::L * P16::T <= 1024 &&
+                      sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget)
+            if (var == 2) return 2 * P16::L;
+    }
+    return P16::L;
+}
+
+template <int N, int... Rs>
+constexpr int row_variant_count()
+{
+    if constexpr (N >= 1024) {
+        using P16 = PlanE<16, N, Rs...>;
+        return (2 * P16::L * P16::T <= 1024 &&
+                sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 3 : 2;
+    }
+    return 1;
+}
+
 template <int G, int N, int... Rs>
-int try_plan(int mode, int n, RowArgs a, hipStream_t s)
+int try_plan(int mode, int var, int n, RowArgs a, hipStream_t s)
 {
     if constexpr (G == IMR_GROUP) {
-        if (n == N) return launch_plan<Plan<N, Rs...>>(mode, a, s);
+        if (n == N) return launch_row_variant<N, Rs...>(var, mode, a, s);
+    }
+    return -1;
+}
+
+template <int G, int N, int... Rs>
+int try_count(int n)
+{
+    if constexpr (G == IMR_GROUP) {
+        if (n == N) return row_variant_count<N, Rs...>();
+    }
+    return -1;
+}
+
+template <int G, int N, int... Rs>
+int try_rows(int n, int var)
+{
+    if constexpr (G == IMR_GROUP) {
+        if (n == N) return row_variant_rows<N, Rs...>(var);
     }
     return -1;
 }
@@ -181,15 +244,39 @@ int try_plan(int mode, int n, RowArgs a, hipStream_t s)
 #define IMR_CAT(a, b) IMR_CAT2(a, b)
 #define IMR_ROW_CASE(g, N, ...)                                                     \
     {                                                                               \
-        const int r = try_plan<g, N, __VA_ARGS__>(mode, n, a, s);                   \
+        const int r = try_plan<g, N, __VA_ARGS__>(mode, var, n, a, s);              \
+        if (r >= 0) return r;                                                       \
+    }
+#define IMR_ROW_COUNT(g, N, ...)                                                    \
+    {                                                                               \
+        const int r = try_count<g, N, __VA_ARGS__>(n);                              \
         if (r >= 0) return r;                                                       \
     }
 
-int IMR_CAT(launch_row_g, IMR_GROUP)(int mode, int n, const LinePlan &lp, RowArgs a, hipStream_t s)
+int IMR_CAT(launch_row_g, IMR_GROUP)(int mode, int var, int n, const LinePlan &lp, RowArgs a,
+                                     hipStream_t s)
 {
     a.tw = lp.tw;
     IMR_PLANS(IMR_ROW_CASE)
     return IMRESH_ERR_UNSUPPORTED_SIZE;
+}
+
+int IMR_CAT(row_variants_g, IMR_GROUP)(int n)
+{
+    IMR_PLANS(IMR_ROW_COUNT)
+    return 0;
+}
+
+#define IMR_ROW_ROWS(g, N, ...)                                                     \
+    {                                                                               \
+        const int r = try_rows<g, N, __VA_ARGS__>(n, var);                          \
+        if (r >= 0) return r;                                                       \
+    }
+
+int IMR_CAT(row_rows_g, IMR_GROUP)(int n, int var)
+{
+    IMR_PLANS(IMR_ROW_ROWS)
+    return 1;
 }
 
 }  // namespace imr

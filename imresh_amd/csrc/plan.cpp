@@ -6,6 +6,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -103,6 +104,26 @@ int get_line_plan(int device, int n, LinePlan *out)
     return IMRESH_OK;
 }
 
+namespace {
+std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tuned;   // (device, W, H) -> variants
+}
+
+void set_tuned_variants(int device, int W, int H, int rowVar, int colVar)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    g_tuned[{device, W, H}] = {rowVar, colVar};
+}
+
+bool get_tuned_variants(int device, int W, int H, int *rowVar, int *colVar)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    auto it = g_tuned.find({device, W, H});
+    if (it == g_tuned.end()) return false;
+    *rowVar = it->second.first;
+    *colVar = it->second.second;
+    return true;
+}
+
 int get_plan2d(int device, int W, int H, Plan2D *out)
 {
     if (W < 1 || H < 1) return IMRESH_ERR_INVALID;
@@ -111,21 +132,35 @@ int get_plan2d(int device, int W, int H, Plan2D *out)
     p.H = H;
     IMR_CHECK(get_line_plan(device, W, &p.row));
     IMR_CHECK(get_line_plan(device, H, &p.col));
+    if (!get_tuned_variants(device, W, H, &p.rowVar, &p.colVar)) p.rowVar = p.colVar = 0;
     *out = p;
     return IMRESH_OK;
 }
 
-int row_grid(const Plan2D &p) { return (p.H + p.row.L - 1) / p.row.L; }
+int row_grid(const Plan2D &p)
+{
+    int L = 1;
+    switch (p.row.group) {
+    case 0: L = row_rows_g0(p.W, p.rowVar); break;
+    case 1: L = row_rows_g1(p.W, p.rowVar); break;
+    case 2: L = row_rows_g2(p.W, p.rowVar); break;
+    case 3: L = row_rows_g3(p.W, p.rowVar); break;
+    case 4: L = row_rows_g4(p.W, p.rowVar); break;
+    default: break;
+    }
+    return (p.H + L - 1) / L;
+}
 
 int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
 {
     a.H = p.H;
+    const int v = p.rowVar;
     switch (p.row.group) {
-    case 0: return launch_row_g0(mode, p.W, p.row, a, s);
-    case 1: return launch_row_g1(mode, p.W, p.row, a, s);
-    case 2: return launch_row_g2(mode, p.W, p.row, a, s);
-    case 3: return launch_row_g3(mode, p.W, p.row, a, s);
-    case 4: return launch_row_g4(mode, p.W, p.row, a, s);
+    case 0: return launch_row_g0(mode, v, p.W, p.row, a, s);
+    case 1: return launch_row_g1(mode, v, p.W, p.row, a, s);
+    case 2: return launch_row_g2(mode, v, p.W, p.row, a, s);
+    case 3: return launch_row_g3(mode, v, p.W, p.row, a, s);
+    case 4: return launch_row_g4(mode, v, p.W, p.row, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
     }
 }
@@ -133,13 +168,38 @@ int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
 int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
 {
     a.W = p.W;
+    const int v = p.colVar;
     switch (p.col.group) {
-    case 0: return launch_col_g0(mode, p.H, p.col, a, s);
-    case 1: return launch_col_g1(mode, p.H, p.col, a, s);
-    case 2: return launch_col_g2(mode, p.H, p.col, a, s);
-    case 3: return launch_col_g3(mode, p.H, p.col, a, s);
-    case 4: return launch_col_g4(mode, p.H, p.col, a, s);
+    case 0: return launch_col_g0(mode, v, p.H, p.col, a, s);
+    case 1: return launch_col_g1(mode, v, p.H, p.col, a, s);
+    case 2: return launch_col_g2(mode, v, p.H, p.col, a, s);
+    case 3: return launch_col_g3(mode, v, p.H, p.col, a, s);
+    case 4: return launch_col_g4(mode, v, p.H, p.col, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
+    }
+}
+
+int row_variants(const LinePlan &lp)
+{
+    switch (lp.group) {
+    case 0: return row_variants_g0(lp.n);
+    case 1: return row_variants_g1(lp.n);
+    case 2: return row_variants_g2(lp.n);
+    case 3: return row_variants_g3(lp.n);
+    case 4: return row_variants_g4(lp.n);
+    default: return 0;
+    }
+}
+
+int col_variants(const LinePlan &lp)
+{
+    switch (lp.group) {
+    case 0: return col_variants_g0(lp.n);
+    case 1: return col_variants_g1(lp.n);
+    case 2: return col_variants_g2(lp.n);
+    case 3: return col_variants_g3(lp.n);
+    case 4: return col_variants_g4(lp.n);
+    default: return 0;
     }
 }
 

@@ -25,12 +25,13 @@ constexpr int kRowThreadsTarget = 256;   // row workgroup size target
 constexpr int kColMaxCols = 8;           // columns per column workgroup (64 B per row)
 constexpr int kLdsBudget = 144 * 1024;   // dynamic LDS per workgroup (of 160 KiB)
 
-template <int N, int... Rs>
-struct Plan {
+// E: complex values per thread and pass (the thread count per line follows)
+template <int E_, int N, int... Rs>
+struct PlanE {
     static constexpr int n = N;
     static constexpr int np = sizeof...(Rs);
     static constexpr int rf[np] = {Rs...};
-    static constexpr int E = kElemsPerThread;
+    static constexpr int E = E_;
 
     __host__ __device__ static constexpr int rad(bool inv, int i) { return inv ? rf[np - 1 - i] : rf[i]; }
     __host__ __device__ static constexpr int ns(bool inv, int i)
@@ -73,6 +74,9 @@ struct Plan {
     static_assert(L * T <= 1024, "row workgroup too large");
     static_assert(C * T <= 1024, "column workgroup too large");
 };
+
+template <int N, int... Rs>
+using Plan = PlanE<kElemsPerThread, N, Rs...>;
 
 // host view of a plan
 struct PlanInfo {

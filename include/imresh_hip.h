@@ -162,6 +162,12 @@ int imresh_fft2d(float *out_dev, const float *in_dev, uint32_t width, uint32_t h
                  int direction, void *stream);
 /* 1 if an FFT plan exists for this line length, else 0. */
 int imresh_fft_supported(uint32_t n);
+/* Picks (once per device and size, by timing) the row- and column-pass
+ * kernel variants used for W x H and reports them.  Called implicitly by the
+ * first reconstruction of a size; no reference counterpart (cuFFT planned
+ * internally, cudaShrinkWrap.cu:226). */
+int imresh_autotune(uint32_t width, uint32_t height, void *stream, int *row_variant,
+                    int *col_variant);
 
 /* ---- task queue (taskQueue.hpp:38-84, taskQueue.cu:91-251) --------------- */
 /* Completion callback: (user, io_host, width, height, status). */
