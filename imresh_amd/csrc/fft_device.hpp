@@ -188,19 +188,34 @@ __device__ __forceinline__ void bfly(cf *v)
 __host__ __device__ constexpr int lpad(int i) { return i + (i >> 5); }
 __host__ __device__ constexpr int lds_elems(int n) { return n + (n >> 5) + 1; }
 
-// One line in LDS: element i at p[lpad(base + i*stride)].  LdsC holds
+// One line in LDS: element e at p[lpad(base + e*STRIDE)].  LdsC holds
 // complex values; LdsS holds one float component at a time (the exchange
 // then runs once for the real and once for the imaginary parts, halving the
-// LDS footprint of a tile).
+// LDS footprint of a tile).  at<STEP>(e0, r) addresses element e0 + r*STEP:
+// when STEP*STRIDE is a multiple of 32 the padded index is affine in r, so
+// the r-offsets become immediate offsets of the LDS instructions.
+template <int STEP, int STRIDE>
+__device__ __forceinline__ int lds_index(int base, int e0, int r)
+{
+    constexpr int d = STEP * STRIDE;
+    const int x0 = base + e0 * STRIDE;
+    if constexpr (d % 32 == 0) return lpad(x0) + r * (d / 32 * 33);
+    else return lpad(x0 + r * d);
+}
+
+template <int STRIDE>
 struct LdsC {
     cf *p;
-    int base, stride;
-    __device__ __forceinline__ cf &operator()(int i) const { return p[lpad(base + i * stride)]; }
+    int base;
+    template <int STEP>
+    __device__ __forceinline__ cf &at(int e0, int r) const { return p[lds_index<STEP, STRIDE>(base, e0, r)]; }
 };
+template <int STRIDE>
 struct LdsS {
     float *p;
-    int base, stride;
-    __device__ __forceinline__ float &operator()(int i) const { return p[lpad(base + i * stride)]; }
+    int base;
+    template <int STEP>
+    __device__ __forceinline__ float &at(int e0, int r) const { return p[lds_index<STEP, STRIDE>(base, e0, r)]; }
 };
 
 // ------------------------------------------------------------ passes
@@ -296,26 +311,53 @@ __device__ __forceinline__ void apply_mid(cf (&v)[P::E], int t, Mid &mid)
 }
 
 // LDS exchange between pass IA of direction DA (its outputs) and pass IB of
-// direction DB (its inputs): write, barrier, read, barrier.
-template <class P, int DA, int IA, int DB, int IB>
-__device__ __forceinline__ void exchange(cf (&v)[P::E], int t, const LdsC &L)
+// direction DB (its inputs): write, barrier, read, barrier.  COMP selects
+// the component moved (0 both as one complex, 1 real part, 2 imaginary).
+template <class P, int DA, int IA, int DB, int IB, int COMP, class Lds>
+__device__ __forceinline__ void exchange_part(cf (&v)[P::E], int t, const Lds &L)
 {
-    for_outputs<P, DA, IA>(t, [&](int s, int e) { L(e) = v[s]; });
+    using A = PassOf<P, DA, IA>;
+    using Bq = PassOf<P, DB, IB>;
+#pragma unroll
+    for (int b = 0; b < A::B; ++b) {
+        const int j = t + b * A::T;
+        if (A::full() || j < A::nb) {
+            const int k = j % A::NS;
+            const int base = (j - k) * A::R + k;
+#pragma unroll
+            for (int r = 0; r < A::R; ++r) {
+                if constexpr (COMP == 0) L.template at<A::NS>(base, r) = v[b * A::R + r];
+                else if constexpr (COMP == 1) L.template at<A::NS>(base, r) = v[b * A::R + r].x;
+                else L.template at<A::NS>(base, r) = v[b * A::R + r].y;
+            }
+        }
+    }
     __syncthreads();
-    for_inputs<P, DB, IB>(t, [&](int s, int e) { v[s] = L(e); });
+#pragma unroll
+    for (int b = 0; b < Bq::B; ++b) {
+        const int j = t + b * Bq::T;
+        if (Bq::full() || j < Bq::nb) {
+#pragma unroll
+            for (int r = 0; r < Bq::R; ++r) {
+                if constexpr (COMP == 0) v[b * Bq::R + r] = L.template at<Bq::nb>(j, r);
+                else if constexpr (COMP == 1) v[b * Bq::R + r].x = L.template at<Bq::nb>(j, r);
+                else v[b * Bq::R + r].y = L.template at<Bq::nb>(j, r);
+            }
+        }
+    }
     __syncthreads();
 }
-template <class P, int DA, int IA, int DB, int IB>
-__device__ __forceinline__ void exchange(cf (&v)[P::E], int t, const LdsS &L)
+
+template <class P, int DA, int IA, int DB, int IB, int S>
+__device__ __forceinline__ void exchange(cf (&v)[P::E], int t, const LdsC<S> &L)
 {
-    for_outputs<P, DA, IA>(t, [&](int s, int e) { L(e) = v[s].x; });
-    __syncthreads();
-    for_inputs<P, DB, IB>(t, [&](int s, int e) { v[s].x = L(e); });
-    __syncthreads();
-    for_outputs<P, DA, IA>(t, [&](int s, int e) { L(e) = v[s].y; });
-    __syncthreads();
-    for_inputs<P, DB, IB>(t, [&](int s, int e) { v[s].y = L(e); });
-    __syncthreads();
+    exchange_part<P, DA, IA, DB, IB, 0>(v, t, L);
+}
+template <class P, int DA, int IA, int DB, int IB, int S>
+__device__ __forceinline__ void exchange(cf (&v)[P::E], int t, const LdsS<S> &L)
+{
+    exchange_part<P, DA, IA, DB, IB, 1>(v, t, L);
+    exchange_part<P, DA, IA, DB, IB, 2>(v, t, L);
 }
 
 // passes [I, np) of direction DIR, each preceded by an LDS exchange

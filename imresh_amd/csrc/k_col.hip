@@ -34,38 +34,44 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
     if (a.done != nullptr && *a.done) return;
     extern __shared__ cf smem[];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
-    const int col = blockIdx.x * C + c;
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
+    // each with its own L2; give every XCD a contiguous range of tiles so
+    // neighbouring tiles (which share 128-byte lines of X and M) meet in the
+    // same L2.
+    int bid = blockIdx.x;
+    if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
+    const int col = bid * C + c;
     const bool valid = col < a.W;
     const int W = a.W;
     // workgroup base (uniform) + 32-bit per-lane offsets e*W + c; columns past
     // the edge read column 0 of the tile and never store
-    const size_t wb = (size_t) blockIdx.x * C;
+    const size_t wb = (size_t) bid * C;
     const int co = valid ? c : 0;
-    using Lds = std::conditional_t<SPLIT, LdsS, LdsC>;
-    const Lds L{(decltype(Lds::p)) smem, c, C};
+    using Lds = std::conditional_t<SPLIT, LdsS<C>, LdsC<C>>;
+    const Lds L{(decltype(Lds::p)) smem, c};
     const cf *tw = (const cf *) a.tw;
     cf v[P::E];
 
     if constexpr (K == CK_FWD) {
         const cf *in = (const cf *) a.in + wb;
         cf *out = (cf *) a.out + wb;
-        auto g0 = [=](int e) { return in[e * W + co]; };
-        auto sl = [=](int e, cf x) { if (valid) out[e * W + co] = x; };
+        auto g0 = [=](int e) { return at32(in, e * W + co); };
+        auto sl = [=](int e, cf x) { if (valid) at32(out, e * W + co) = x; };
         line_fft<P, -1>(v, tw, t, L, g0, sl);
     } else if constexpr (K == CK_INV) {
         const bool real_in = (a.flags & kColRealIn) != 0;
         const cf *in = (const cf *) a.in + wb;
         const float *M = a.M + wb;
         cf *out = (cf *) a.out + wb;
-        auto g0 = [=](int e) { return real_in ? mkc(M[e * W + co], 0.f) : in[e * W + co]; };
-        auto sl = [=](int e, cf x) { if (valid) out[e * W + co] = x; };
+        auto g0 = [=](int e) { return real_in ? mkc(at32(M, e * W + co), 0.f) : at32(in, e * W + co); };
+        auto sl = [=](int e, cf x) { if (valid) at32(out, e * W + co) = x; };
         line_fft<P, +1>(v, tw, t, L, g0, sl);
     } else if constexpr (K == CK_PROJ) {
         cf *X = (cf *) a.X + wb;
         const float *M = a.M + wb;
-        auto g0 = [=](int e) { return X[e * W + co]; };
-        auto mid = [=](int e, cf z) { return modulus_project(z, M[e * W + co]); };
-        auto sl = [=](int e, cf x) { if (valid) X[e * W + co] = x; };
+        auto g0 = [=](int e) { return at32(X, e * W + co); };
+        auto mid = [=](int e, cf z) { return modulus_project_fused(z, at32(M, e * W + co)); };
+        auto sl = [=](int e, cf x) { if (valid) at32(X, e * W + co) = x; };
         line_pair<P, -1, +1>(v, tw, t, L, g0, mid, sl);
     }
 }

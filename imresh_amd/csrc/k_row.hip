@@ -42,18 +42,18 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     // read row 0 of the workgroup and never store
     const size_t wb = (size_t) blockIdx.x * NL * n;
     const int lo = valid ? l * n : 0;
-    const LdsC L{smem, l * n, 1};
+    const LdsC<1> L{smem, l * n};
     const cf *tw = (const cf *) a.tw;
     cf v[P::E];
     cf *X = (cf *) a.X + wb;
-    auto ldX = [=](int e) { return X[lo + e]; };
-    auto stX = [=](int e, cf x) { if (valid) X[lo + e] = x; };
+    auto ldX = [=](int e) { return at32(X, lo + e); };
+    auto stX = [=](int e, cf x) { if (valid) at32(X, lo + e) = x; };
 
     if constexpr (K == RK_FWD || K == RK_INV) {
         const cf *in = (const cf *) a.in + wb;
         cf *out = (cf *) a.out + wb;
-        auto g0 = [=](int e) { return in[lo + e]; };
-        auto sl = [=](int e, cf x) { if (valid) out[lo + e] = x; };
+        auto g0 = [=](int e) { return at32(in, lo + e); };
+        auto sl = [=](int e, cf x) { if (valid) at32(out, lo + e) = x; };
         if constexpr (K == RK_FWD) line_fft<P, -1>(v, tw, t, L, g0, sl);
         else line_fft<P, +1>(v, tw, t, L, g0, sl);
     } else if constexpr (K == RK_INVOUT) {
@@ -67,12 +67,12 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         unsigned cnt = 0;
         auto sl = [&](int e, cf g) {
             if (!valid) return;
-            X[lo + e] = g;
-            if (fl & kRowWriteG) G[lo + e] = g;
+            at32(X, lo + e) = g;
+            if (fl & kRowWriteG) at32(G, lo + e) = g;
             if (fl & (kRowWriteBn | kRowError)) {
                 const float nrm = cnorm(g);
-                if (fl & kRowWriteBn) Bn[lo + e] = nrm;
-                if ((fl & kRowError) && mask[lo + e]) {
+                if (fl & kRowWriteBn) at32(Bn, lo + e) = nrm;
+                if ((fl & kRowError) && at32(mask, lo + e)) {
                     err += nrm;
                     cnt += 1;
                 }
@@ -95,16 +95,16 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         const float *B2 = a.B2 + wb;
         const float beta = a.beta;
         auto g0 = [=](int e) {
-            const cf g = X[lo + e];
+            const cf g = at32(X, lo + e);
             bool m;
             if (cut) {
-                m = B2[lo + e] < thr;
-                if (valid) mask[lo + e] = m ? 1 : 0;
+                m = at32(B2, lo + e) < thr;
+                if (valid) at32(mask, lo + e) = m ? 1 : 0;
             } else {
-                m = mask[lo + e] == 1;
+                m = at32(mask, lo + e) == 1;
             }
-            const cf gp = hio_constraint(G[lo + e], g, m, beta);
-            if (valid) G[lo + e] = gp;
+            const cf gp = hio_constraint(at32(G, lo + e), g, m, beta);
+            if (valid) at32(G, lo + e) = gp;
             return gp;
         };
         line_fft<P, -1>(v, tw, t, L, g0, stX);
@@ -113,8 +113,8 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         const uint8_t *mask = a.mask + wb;
         const float beta = a.beta;
         auto mid = [=](int e, cf g) {
-            const cf gp = hio_constraint(G[lo + e], g, mask[lo + e] == 1, beta);
-            if (valid) G[lo + e] = gp;
+            const cf gp = hio_constraint(at32(G, lo + e), g, at32(mask, lo + e) == 1, beta);
+            if (valid) at32(G, lo + e) = gp;
             return gp;
         };
         line_pair<P, +1, -1>(v, tw, t, L, ldX, mid, stX);

@@ -33,6 +33,37 @@ __device__ __forceinline__ cf modulus_project(cf z, float m)
     return mkc(z.x * f, z.y * f);
 }
 
+// Global access at a workgroup-uniform base plus a 32-bit element index:
+// base + zext(idx * sizeof(T)) maps onto the scalar-base + 32-bit vector
+// offset addressing of global loads/stores (callers keep idx * sizeof(T)
+// below 4 GiB: at most 8192 x 8192 complex values).
+template <class T>
+__device__ __forceinline__ T &at32(T *base, unsigned idx)
+{
+    return *(T *) ((char *) base + (unsigned) (idx * (unsigned) sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ const T &at32(const T *base, unsigned idx)
+{
+    return *(const T *) ((const char *) base + (unsigned) (idx * (unsigned) sizeof(T)));
+}
+
+// Projection used inside the fused column pass: f = m * rsqrt(|z|^2) with
+// the hardware reciprocal square root (1 ulp) instead of the correctly
+// rounded sqrtf and division (~20 instructions); within 2 ulp of
+// modulus_project, far below the FFT's own rounding.  Norms below 1e-15
+// (|z|^2 < 1e-30, near the denormal range) take the exact path, which also
+// keeps z == 0 at 0.
+__device__ __forceinline__ cf modulus_project_fused(cf z, float m)
+{
+    const float n2 = z.x * z.x + z.y * z.y;
+    if (n2 >= 1e-30f) {
+        const float f = m * __builtin_amdgcn_rsqf(n2);
+        return mkc(z.x * f, z.y * f);
+    }
+    return modulus_project(z, m);
+}
+
 // deterministic workgroup reduction through LDS (any blockDim <= 1024):
 // a fixed pairwise tree, so results do not depend on scheduling
 template <class T, class Op>
