@@ -54,15 +54,18 @@ def log(msg: str) -> None:
 
 
 def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch measured with rocprofv3 (profiles/traffic_<kernel>_2048.json)."""
-    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_2048.json")
+    """HBM bytes per launch of `kernel` at 2048^2 measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE, scaled by a copy-kernel calibration
+    (profiles/traffic_shrinkwrap_2048.md, lines `traffic <kernel> <bytes>`)."""
+    p = os.path.join(ROOT, "profiles", "traffic_shrinkwrap_2048.md")
     if not os.path.exists(p):
         return None
-    try:
-        with open(p) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except Exception:
-        return None
+    with open(p) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                return float(parts[2])
+    return None
 
 
 def cpu_baseline(M: np.ndarray, budget_s: float = 12.0, max_iter: int = 20) -> dict:

@@ -47,6 +47,11 @@ class Result(ctypes.Structure):
                 ("last_masked", c_float)]
 
 
+class Image(ctypes.Structure):
+    """imresh_image: one image of a batch (data: host or device pointer)."""
+    _fields_ = [("data", c_void_p), ("width", c_uint32), ("height", c_uint32)]
+
+
 TASK_CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, POINTER(c_float), c_uint32, c_uint32, c_int)
 
 _LIB = None
@@ -67,6 +72,10 @@ _SIGS = {
     "imresh_shrinkwrap": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), _P, POINTER(Result)]),
     "imresh_shrinkwrap_device": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), _P,
                                          POINTER(Result)]),
+    "imresh_shrinkwrap_batch": (c_int, [POINTER(Image), c_uint32, POINTER(Params), _P,
+                                        POINTER(Result)]),
+    "imresh_shrinkwrap_batch_device": (c_int, [POINTER(Image), c_uint32, POINTER(Params), _P,
+                                               POINTER(Result)]),
     "imresh_hio_iterations": (c_int, [_P, _P, _P, _P, c_uint32, c_uint32, c_float, c_uint32, _P]),
     "imresh_copy_to_real_part": (c_int, [_P, _P, c_uint32, _P]),
     "imresh_copy_from_real_part": (c_int, [_P, _P, c_uint32, _P]),
@@ -214,6 +223,36 @@ def shrink_wrap_device(io_dev: int, width: int, height: int, params: Params | No
                                           ctypes.byref(r) if want_result else None),
           "imresh_shrinkwrap_device")
     return r if want_result else None
+
+
+def shrink_wrap_batch(ios: list, params: Params | None = None, stream=None) -> list:
+    """Reconstructs a list of float32 H x W arrays (each |F| in, Re(g') out,
+    in place) together; returns one Result per image."""
+    for io in ios:
+        if io.dtype != np.float32 or not io.flags.c_contiguous or io.ndim != 2:
+            raise ValueError("every image must be a C-contiguous float32 H x W array")
+    p = params if params is not None else default_params()
+    imgs = (Image * max(1, len(ios)))()
+    for i, io in enumerate(ios):
+        imgs[i] = Image(io.ctypes.data, io.shape[1], io.shape[0])
+    res = (Result * max(1, len(ios)))()
+    check(_lib().imresh_shrinkwrap_batch(imgs, len(ios), ctypes.byref(p), stream, res),
+          "imresh_shrinkwrap_batch")
+    return [res[i] for i in range(len(ios))]
+
+
+def shrink_wrap_batch_device(images: list, params: Params | None = None, stream=None,
+                             want_results: bool = True):
+    """Batch over device buffers: images = [(device_ptr, width, height), ...]."""
+    p = params if params is not None else default_params()
+    imgs = (Image * max(1, len(images)))()
+    for i, (ptr, w, h) in enumerate(images):
+        imgs[i] = Image(ptr, w, h)
+    res = (Result * max(1, len(images)))()
+    check(_lib().imresh_shrinkwrap_batch_device(imgs, len(images), ctypes.byref(p), stream,
+                                                res if want_results else None),
+          "imresh_shrinkwrap_batch_device")
+    return [res[i] for i in range(len(images))] if want_results else None
 
 
 _TASK_REFS: dict = {}

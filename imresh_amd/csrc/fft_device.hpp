@@ -244,9 +244,36 @@ __device__ __forceinline__ void gather_global(cf (&v)[P::E], int t, Src &src)
     }
 }
 
+// twiddles of pass I for this thread's butterflies (none for a first pass)
+template <class P, int DIR, int I>
+struct TwRegs {
+    using Q = PassOf<P, DIR, I>;
+    static constexpr int count = Q::NS > 1 ? Q::B * (Q::R - 1) : 0;
+    cf w[count > 0 ? count : 1];
+};
+
+// global (L1/L2-resident) twiddle loads of pass I; issued before the LDS
+// exchange that precedes the pass so their latency overlaps the barrier
+template <class P, int DIR, int I>
+__device__ __forceinline__ void load_tw(TwRegs<P, DIR, I> &w, const cf *__restrict__ tw, int t)
+{
+    using Q = PassOf<P, DIR, I>;
+    if constexpr (Q::NS > 1) {
+#pragma unroll
+        for (int b = 0; b < Q::B; ++b) {
+            const int j = t + b * Q::T;
+            if (Q::full() || j < Q::nb) {
+                const cf *p = tw + Q::TWOFF + (j % Q::NS) * (Q::R - 1);
+#pragma unroll
+                for (int r = 1; r < Q::R; ++r) w.w[b * (Q::R - 1) + r - 1] = p[r - 1];
+            }
+        }
+    }
+}
+
 // twiddle multiply + radix-R butterflies of pass I, in registers
 template <class P, int DIR, int I>
-__device__ __forceinline__ void compute(cf (&v)[P::E], const cf *__restrict__ tw, int t)
+__device__ __forceinline__ void compute_tw(cf (&v)[P::E], const TwRegs<P, DIR, I> &w, int t)
 {
     using Q = PassOf<P, DIR, I>;
 #pragma unroll
@@ -254,14 +281,21 @@ __device__ __forceinline__ void compute(cf (&v)[P::E], const cf *__restrict__ tw
         const int j = t + b * Q::T;
         if (Q::full() || j < Q::nb) {
             if constexpr (Q::NS > 1) {
-                const int k = j % Q::NS;
-                const cf *w = tw + Q::TWOFF + k * (Q::R - 1);
 #pragma unroll
-                for (int r = 1; r < Q::R; ++r) v[b * Q::R + r] = twmul<DIR>(v[b * Q::R + r], w[r - 1]);
+                for (int r = 1; r < Q::R; ++r)
+                    v[b * Q::R + r] = twmul<DIR>(v[b * Q::R + r], w.w[b * (Q::R - 1) + r - 1]);
             }
             bfly<Q::R, DIR>(&v[b * Q::R]);
         }
     }
+}
+
+template <class P, int DIR, int I>
+__device__ __forceinline__ void compute(cf (&v)[P::E], const cf *__restrict__ tw, int t)
+{
+    TwRegs<P, DIR, I> w;
+    load_tw<P, DIR, I>(w, tw, t);
+    compute_tw<P, DIR, I>(v, w, t);
 }
 
 // output positions of pass I: (j - j%ns)*R + j%ns + r*ns
@@ -311,13 +345,17 @@ __device__ __forceinline__ void apply_mid(cf (&v)[P::E], int t, Mid &mid)
 }
 
 // LDS exchange between pass IA of direction DA (its outputs) and pass IB of
-// direction DB (its inputs): write, barrier, read, barrier.  COMP selects
-// the component moved (0 both as one complex, 1 real part, 2 imaginary).
+// direction DB (its inputs): barrier, write, barrier, read.  The leading
+// barrier orders these writes after every read of the previous exchange, so
+// a wave that has read its inputs goes on to compute without waiting for
+// the others.  COMP selects the component moved (0 both as one complex,
+// 1 real part, 2 imaginary).
 template <class P, int DA, int IA, int DB, int IB, int COMP, class Lds>
 __device__ __forceinline__ void exchange_part(cf (&v)[P::E], int t, const Lds &L)
 {
     using A = PassOf<P, DA, IA>;
     using Bq = PassOf<P, DB, IB>;
+    __syncthreads();
 #pragma unroll
     for (int b = 0; b < A::B; ++b) {
         const int j = t + b * A::T;
@@ -345,7 +383,6 @@ __device__ __forceinline__ void exchange_part(cf (&v)[P::E], int t, const Lds &L
             }
         }
     }
-    __syncthreads();
 }
 
 template <class P, int DA, int IA, int DB, int IB, int S>
@@ -365,8 +402,10 @@ template <class P, int DIR, int I, class Lds>
 __device__ __forceinline__ void passes_from(cf (&v)[P::E], const cf *tw, int t, const Lds &L)
 {
     if constexpr (I < P::np) {
+        TwRegs<P, DIR, I> w;
+        load_tw<P, DIR, I>(w, tw, t);
         exchange<P, DIR, I - 1, DIR, I>(v, t, L);
-        compute<P, DIR, I>(v, tw, t);
+        compute_tw<P, DIR, I>(v, w, t);
         passes_from<P, DIR, I + 1>(v, tw, t, L);
     }
 }
@@ -386,17 +425,25 @@ __device__ __forceinline__ void line_fft(cf (&v)[P::E], const cf *tw, int t, con
 // then the transform in direction DB.  A's last pass and B's first pass
 // have the same radix (mirrored sequences), so the junction never leaves
 // registers: B's first pass consumes exactly the positions A's last pass
-// produced in the same thread.
-template <class P, int DA, int DB, class Lds, class Src, class Mid, class Dst>
+// produced in the same thread.  `pre(e)` runs right after the input gather,
+// so operands of `mid` read from global memory (pre returns them; `mid(e, z,
+// p)` receives them) are in flight during the first transform instead of
+// stalling the junction; a `pre` returning NoPre leaves the loads to `mid`.
+struct NoPre {};
+
+template <class P, int DA, int DB, class Lds, class Src, class Pre, class Mid, class Dst>
 __device__ __forceinline__ void line_pair(cf (&v)[P::E], const cf *tw, int t, const Lds &L, Src &src,
-                                          Mid &mid, Dst &dst)
+                                          Pre &pre, Mid &mid, Dst &dst)
 {
     static_assert(DA == -DB, "a pair runs opposite directions");
     static_assert(P::rad(DA > 0, P::np - 1) == P::rad(DB > 0, 0), "mirrored junction radix");
+    using PT = decltype(pre(0));
+    PT pf[P::E];
     gather_global<P, DA, 0>(v, t, src);
+    for_outputs<P, DA, P::np - 1>(t, [&](int s, int e) { pf[s] = pre(e); });
     compute<P, DA, 0>(v, tw, t);
     passes_from<P, DA, 1>(v, tw, t, L);
-    apply_mid<P, DA, P::np - 1>(v, t, mid);
+    for_outputs<P, DA, P::np - 1>(t, [&](int s, int e) { v[s] = mid(e, v[s], pf[s]); });
     compute<P, DB, 0>(v, tw, t);
     passes_from<P, DB, 1>(v, tw, t, L);
     scatter_global<P, DB, P::np - 1>(v, t, dst);

@@ -45,8 +45,10 @@ struct RowArgs {
     unsigned *pcnt;        // per-workgroup sum mask
     const int *done;       // early-exit flag (NULL: never)
     int flags;             // kRow* bits (set by launch_row from the mode)
-    int H;                 // number of rows
+    int H;                 // number of rows (of one image)
     const float2 *tw;      // row-length twiddle table (set by launch_row)
+    int batch;             // images (grid.y; 0 or 1: one image)
+    size_t istride;        // elements between consecutive images
 };
 
 struct ColArgs {
@@ -58,6 +60,8 @@ struct ColArgs {
     int flags;             // kCol* bits (set by launch_col from the mode)
     int W;                 // number of columns = row pitch
     const float2 *tw;      // column-length twiddle table (set by launch_col)
+    int batch;             // images (grid.y; 0 or 1: one image)
+    size_t istride;        // elements between consecutive images
 };
 
 struct BlurArgs {
@@ -70,7 +74,18 @@ struct BlurArgs {
     int W, H;
     float *pmax;           // per-workgroup max (NULL: none)
     const int *done;
+    int batch;             // images (grid.z; 0 or 1: one image)
+    size_t istride;        // elements between consecutive images
+    // non-NULL: the weights of image b are wh/wv + kBlurWeightRow * cycles[b],
+    // cycles[b] = completed shrink-wrap cycles (one weights row per cycle)
+    const unsigned *cycles;
 };
+
+// per-image device scalars of a batch: kScalStride floats {threshold, error,
+// total, masked} and kFlagStride ints {done, cycles}
+constexpr int kScalStride = 4;
+constexpr int kFlagStride = 2;
+constexpr int kBlurWeightRow = 64;
 
 // A compile-time plan (plans.hpp) of one line length with its twiddles.
 struct LinePlan {
@@ -118,11 +133,17 @@ int row_grid(const Plan2D &p);
 int blur_grid(int W, int H);
 int launch_blur(const BlurArgs &a, hipStream_t s);
 int launch_blur_h_inplace(float *d, int W, int H, const float *w, int r, hipStream_t s);
+int launch_max_finalize_batch(const float *pmax, int np, float cutA, float cut, float *scal,
+                              int *flags, int batch, hipStream_t s);
+int launch_err_finalize_batch(const float *perr, const unsigned *pcnt, int np, float target,
+                              float *scal, int *flags, int batch, hipStream_t s);
 int launch_max_finalize(const float *pmax, int np, float cut, float *thr, const int *done,
                         hipStream_t s);
 int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float target,
                         float *res, int *done, unsigned *cycles, hipStream_t s);
 int launch_copy_from_real(float *out, const float2 *in, size_t n, hipStream_t s);
+int launch_gather_images(float *dst, const float *const *src, size_t n, int batch, hipStream_t s);
+int launch_scatter_real(float *const *dst, const float2 *X, size_t n, int batch, hipStream_t s);
 int launch_copy_to_real(float2 *out, const float *in, size_t n, hipStream_t s);
 int launch_complex_norm(float *out, const float2 *in, size_t n, hipStream_t s);
 int launch_apply_modulus(float2 *out, const float2 *in, const float *M, size_t n, hipStream_t s);
@@ -139,7 +160,8 @@ int launch_fill_state(float2 *X, float2 *G, uint8_t *mask, float *M, float *Bn, 
 
 constexpr int kReduceBlocks = 512;
 constexpr int kBlurTX = 64;
-constexpr int kBlurTY = 32;
+constexpr int kBlurTY = 64;
+constexpr int kMaxBlurRadius = 31;   // 63 taps
 
 // host Gaussian weights: calcGaussianKernel semantics (calcGaussianKernel.cpp:42-91)
 int gaussian_weights(double sigma, float *w, unsigned capacity);

@@ -27,11 +27,14 @@ namespace {
 enum ColKernel { CK_FWD, CK_INV, CK_PROJ };
 
 // C columns per workgroup; SPLIT: exchange real and imaginary parts through
-// LDS one after the other (half the LDS of a complex tile).
-template <class P, int C, bool SPLIT, int K>
+// LDS one after the other (half the LDS of a complex tile); PF (CK_PROJ
+// only): the measured modulus is read right after the column gather instead
+// of at the junction.
+template <class P, int C, bool SPLIT, int K, bool PF = false>
 __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 {
-    if (a.done != nullptr && *a.done) return;
+    const int b = blockIdx.y;   // image of a batch
+    if (a.done != nullptr && a.done[b * kFlagStride]) return;
     extern __shared__ cf smem[];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
@@ -45,7 +48,7 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
     const int W = a.W;
     // workgroup base (uniform) + 32-bit per-lane offsets e*W + c; columns past
     // the edge read column 0 of the tile and never store
-    const size_t wb = (size_t) bid * C;
+    const size_t wb = (size_t) b * a.istride + (size_t) bid * C;
     const int co = valid ? c : 0;
     using Lds = std::conditional_t<SPLIT, LdsS<C>, LdsC<C>>;
     const Lds L{(decltype(Lds::p)) smem, c};
@@ -70,9 +73,16 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
         cf *X = (cf *) a.X + wb;
         const float *M = a.M + wb;
         auto g0 = [=](int e) { return at32(X, e * W + co); };
-        auto mid = [=](int e, cf z) { return modulus_project_fused(z, at32(M, e * W + co)); };
+        auto pre = [=](int e) {
+            if constexpr (PF) return at32(M, e * W + co);
+            else return NoPre{};
+        };
+        auto mid = [=](int e, cf z, auto m) {
+            if constexpr (PF) return modulus_project_fused(z, m);
+            else return modulus_project_fused(z, at32(M, e * W + co));
+        };
         auto sl = [=](int e, cf x) { if (valid) at32(X, e * W + co) = x; };
-        line_pair<P, -1, +1>(v, tw, t, L, g0, mid, sl);
+        line_pair<P, -1, +1>(v, tw, t, L, g0, pre, mid, sl);
     }
 }
 
@@ -80,19 +90,19 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 template <class P, int C, bool SPLIT>
 constexpr size_t col_lds() { return (SPLIT ? sizeof(float) : sizeof(cf)) * (size_t) lds_elems(C * P::n); }
 
-template <class P, int C, bool SPLIT>
+template <class P, int C, bool SPLIT, bool PF = false>
 int launch_variant(int mode, ColArgs a, hipStream_t s)
 {
     static_assert(C * P::T <= 1024, "column workgroup too large");
     static_assert(col_lds<P, C, SPLIT>() <= kLdsBudget, "column tile exceeds the LDS budget");
-    const dim3 grid((a.W + C - 1) / C), block(C * P::T);
+    const dim3 grid((a.W + C - 1) / C, a.batch > 1 ? a.batch : 1), block(C * P::T);
     constexpr size_t lds = col_lds<P, C, SPLIT>();
     if constexpr (lds > 64 * 1024) {
         static const hipError_t attr = [&] {
             hipError_t e = hipSuccess;
             for (const void *f : {(const void *) k_col<P, C, SPLIT, CK_FWD>,
                                   (const void *) k_col<P, C, SPLIT, CK_INV>,
-                                  (const void *) k_col<P, C, SPLIT, CK_PROJ>})
+                                  (const void *) k_col<P, C, SPLIT, CK_PROJ, PF>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -116,37 +126,50 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
         break;
     case COL_PROJ:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ, PF>), grid, block, lds, s, a);
         break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());
 }
 
-// Column variants of a length: 0 = E 16 with the plan's default tile; for
-// long columns also E 32 (half the threads per column, 512-thread groups)
-// and E 32 with 16-column split-LDS tiles (128-byte row segments).
+// Column variants of a length (index: tile shape, then +count/2 for the
+// prefetching CK_PROJ): shapes 0 = E 16 with the plan's default tile; for
+// long columns also 1 = E 32 (half the threads per column, 512-thread
+// groups) and 2 = E 32 with 16-column split-LDS tiles (128-byte row
+// segments).
 template <int N, int... Rs>
-int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
-{
-    using P16 = PlanE<16, N, Rs...>;
-    if constexpr (N >= 1024) {
-        using P32 = PlanE<32, N, Rs...>;
-        if (var == 1) return launch_variant<P32, P32::C, false>(mode, a, s);
-        if constexpr (16 * P32::T <= 1024 && col_lds<P32, 16, true>() <= kLdsBudget)
-            if (var == 2) return launch_variant<P32, 16, true>(mode, a, s);
-    }
-    return launch_variant<P16, P16::C, false>(mode, a, s);
-}
-
-template <int N, int... Rs>
-constexpr int col_variant_count()
+constexpr int col_shape_count()
 {
     if constexpr (N >= 1024) {
         using P32 = PlanE<32, N, Rs...>;
         return (16 * P32::T <= 1024 && col_lds<P32, 16, true>() <= kLdsBudget) ? 3 : 2;
     }
     return 1;
+}
+
+template <int N, int... Rs>
+constexpr int col_variant_count() { return 2 * col_shape_count<N, Rs...>(); }
+
+template <int N, int... Rs>
+int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
+{
+    using P16 = PlanE<16, N, Rs...>;
+    constexpr int ns = col_shape_count<N, Rs...>();
+    const bool pf = var >= ns;
+    const int shape = pf ? var - ns : var;
+    if constexpr (N >= 1024) {
+        using P32 = PlanE<32, N, Rs...>;
+        if (shape == 1)
+            return pf ? launch_variant<P32, P32::C, false, true>(mode, a, s)
+                      : launch_variant<P32, P32::C, false>(mode, a, s);
+        if constexpr (ns == 3)
+            if (shape == 2)
+                return pf ? launch_variant<P32, 16, true, true>(mode, a, s)
+                          : launch_variant<P32, 16, true>(mode, a, s);
+    }
+    return pf ? launch_variant<P16, P16::C, false, true>(mode, a, s)
+              : launch_variant<P16, P16::C, false>(mode, a, s);
 }
 
 template <int G, int N, int... Rs>

@@ -5,7 +5,9 @@
 // elementwise operators as standalone kernels.  Blur taps are summed from 0
 // in the reference order w[0]*x[i-r] ... w[2r]*x[i+r] (gaussian.cpp:51-163,
 // 559-734; cudaGaussian.cu:539-652, 776-990) with clamp-to-edge borders.
+#include <algorithm>
 #include <cfloat>
+#include <utility>
 
 #include "internal.hpp"
 #include "ops_device.hpp"
@@ -13,65 +15,125 @@
 namespace imr {
 
 // ------------------------------------------------------------ Gaussian blur
-// Out-of-place separable blur of a kBlurTX x kBlurTY output tile: the input
-// tile with a clamped halo is staged in LDS, the horizontal pass runs over
-// all halo rows, then the vertical pass; each output is summed in the
-// reference tap order.  Optionally writes the tile maximum.
+// Out-of-place separable blur of a kBlurTX x kBlurTY output tile for a tap
+// radius R known at compile time (every weight stays in registers):
+//   1. the input tile with a clamped R-wide halo is staged in LDS;
+//   2. horizontal pass in place in LDS: 16 lanes per row, each reading a
+//      (4 + 2R)-wide window with 16-byte LDS reads and writing 4 outputs;
+//      a row is owned by one wave, so its reads precede its writes;
+//   3. vertical pass: each lane keeps a (16 + 2R)-long column window in
+//      registers and writes 16 outputs, coalesced along the rows;
+//   4. optional tile maximum of the outputs (wave64 DPP reduction).
+// Each output is summed in the reference tap order from 0.  The identity
+// flags turn a pass into a copy (the one-directional blur entry points).
+template <int R>
 __global__ void __launch_bounds__(256) k_blur2d(BlurArgs a)
 {
-    if (a.done != nullptr && *a.done) return;
-    extern __shared__ float sm[];
-    __shared__ float red[1024];
-    const int rh = a.rh, rv = a.rv;
-    const int AW = kBlurTX + 2 * rh, AH = kBlurTY + 2 * rv;
-    float *wh = sm;                  // 64
-    float *wv = sm + 64;             // 64
-    float *A = sm + 128;             // AH x AW
-    float *Hs = A + AH * AW;         // AH x kBlurTX
-    const int x0 = blockIdx.x * kBlurTX, y0 = blockIdx.y * kBlurTY;
-    const int tid = threadIdx.x;
-    for (int i = tid; i < 2 * rh + 1; i += blockDim.x) wh[i] = a.wh[i];
-    for (int i = tid; i < 2 * rv + 1; i += blockDim.x) wv[i] = a.wv[i];
-    for (int i = tid; i < AH * AW; i += blockDim.x) {
-        const int ty = i / AW, tx = i - ty * AW;
-        const int gy = min(max(y0 - rv + ty, 0), a.H - 1);
-        const int gx = min(max(x0 - rh + tx, 0), a.W - 1);
-        A[i] = a.in[(size_t) gy * a.W + gx];
+    const int b = blockIdx.z;   // image of a batch
+    if (a.done != nullptr && a.done[b * kFlagStride]) return;
+    constexpr int TX = kBlurTX, TY = kBlurTY, NT = 2 * R + 1;
+    constexpr int AW = TX + 2 * R, AH = TY + 2 * R, AWP = (AW + 3) & ~3;
+    constexpr int NV = (4 + 2 * R + 3) / 4;   // 16-byte reads per row window
+    constexpr int RPT = TY / 4;               // output rows per lane
+    static_assert(TX == 64 && TY % 4 == 0, "tile shape");
+    extern __shared__ float4 blur_sm[];
+    float *A = reinterpret_cast<float *>(blur_sm);
+    __shared__ float red[4];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, W = a.W, H = a.H;
+    const size_t wrow = a.cycles != nullptr ? (size_t) kBlurWeightRow * a.cycles[b * kFlagStride] : 0;
+    const float *in = a.in + (size_t) b * a.istride;
+    float *out = a.out + (size_t) b * a.istride;
+    float wh[NT], wv[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        wh[k] = a.wh[wrow + k];
+        wv[k] = a.wv[wrow + k];
+    }
+    {
+        // all global loads of the staged tile are issued before the LDS
+        // stores; rows/columns past the tile are clamped reads, not stored
+        constexpr int NRW = (AH + 3) / 4, NCX = (AW + 63) / 64;
+        float buf[NRW][NCX];
+#pragma unroll
+        for (int i = 0; i < NRW; ++i) {
+            const int gy = min(max(y0 - R + wave + 4 * i, 0), H - 1);
+            const float *src = in + (size_t) gy * W;
+#pragma unroll
+            for (int j = 0; j < NCX; ++j)
+                buf[i][j] = src[min(max(x0 - R + lane + 64 * j, 0), W - 1)];
+        }
+#pragma unroll
+        for (int i = 0; i < NRW; ++i) {
+            const int ry = wave + 4 * i;
+#pragma unroll
+            for (int j = 0; j < NCX; ++j) {
+                const int cx = lane + 64 * j;
+                if (ry < AH && cx < AW) A[ry * AWP + cx] = buf[i][j];
+            }
+        }
     }
     __syncthreads();
-    for (int i = tid; i < AH * kBlurTX; i += blockDim.x) {
-        const int ty = i / kBlurTX, tx = i - ty * kBlurTX;
-        const float *x = A + ty * AW + tx;
-        float sum;
-        if (a.h_identity) {
-            sum = x[rh];
-        } else {
-            sum = 0.f;
-            for (int k = 0; k <= 2 * rh; ++k) sum += x[k] * wh[k];
+    {
+        const int q = 4 * (t & 15);
+        for (int ry = t >> 4; ry < AH; ry += 16) {
+            float *row = A + ry * AWP;
+            float o[4];
+            if (a.h_identity) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) o[k] = row[q + R + k];
+            } else {
+                float v[4 * NV];
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    const float4 f = *reinterpret_cast<const float4 *>(row + q + 4 * i);
+                    v[4 * i] = f.x;
+                    v[4 * i + 1] = f.y;
+                    v[4 * i + 2] = f.z;
+                    v[4 * i + 3] = f.w;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    float sum = 0.f;
+#pragma unroll
+                    for (int j = 0; j < NT; ++j) sum += v[k + j] * wh[j];
+                    o[k] = sum;
+                }
+            }
+            *reinterpret_cast<float4 *>(row + q) = make_float4(o[0], o[1], o[2], o[3]);
         }
-        Hs[i] = sum;
     }
     __syncthreads();
     float lmax = -FLT_MAX;
-    for (int i = tid; i < kBlurTY * kBlurTX; i += blockDim.x) {
-        const int ty = i / kBlurTX, tx = i - ty * kBlurTX;
-        const int gy = y0 + ty, gx = x0 + tx;
-        if (gy < a.H && gx < a.W) {
-            const float *x = Hs + ty * kBlurTX + tx;
+    {
+        const int r0 = wave * RPT, gx = x0 + lane;
+        float v[RPT + 2 * R];
+#pragma unroll
+        for (int j = 0; j < RPT + 2 * R; ++j) v[j] = A[(r0 + j) * AWP + lane];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
             float sum;
             if (a.v_identity) {
-                sum = x[rv * kBlurTX];
+                sum = v[k + R];
             } else {
                 sum = 0.f;
-                for (int k = 0; k <= 2 * rv; ++k) sum += x[k * kBlurTX] * wv[k];
+#pragma unroll
+                for (int j = 0; j < NT; ++j) sum += v[k + j] * wv[j];
             }
-            a.out[(size_t) gy * a.W + gx] = sum;
-            lmax = fmax_ref(lmax, sum);
+            const int gy = y0 + r0 + k;
+            if (gx < W && gy < H) {
+                out[(size_t) gy * W + gx] = sum;
+                lmax = fmax_ref(lmax, sum);
+            }
         }
     }
     if (a.pmax != nullptr) {
-        lmax = block_reduce(lmax, red, [](float m, float x) { return fmax_ref(m, x); });
-        if (tid == 0) a.pmax[blockIdx.y * gridDim.x + blockIdx.x] = lmax;
+        lmax = wave_reduce(lmax, [](float m, float x) { return fmax_ref(m, x); });
+        if (lane == 0) red[wave] = lmax;
+        __syncthreads();
+        if (t == 0)
+            a.pmax[((size_t) b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+                fmax_ref(fmax_ref(red[0], red[1]), fmax_ref(red[2], red[3]));
     }
 }
 
@@ -110,6 +172,24 @@ __global__ void __launch_bounds__(1024) k_max_finalize(const float *pmax, int np
         float t = m;
         t *= cut;
         *thr = t;
+    }
+}
+
+// batched form: workgroup b finalizes image b; the cut-off factor is cutA
+// in the image's first cycle and cut afterwards (cudaShrinkWrap.cu:275-276)
+__global__ void __launch_bounds__(1024) k_max_finalize_batch(const float *pmax, int np, float cutA,
+                                                           float cut, float *scal, const int *flags)
+{
+    const int b = blockIdx.x;
+    if (flags[b * kFlagStride]) return;
+    __shared__ float red[1024];
+    float m = -FLT_MAX;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) m = fmax_ref(m, pmax[(size_t) b * np + i]);
+    m = block_reduce(m, red, [](float x, float y) { return fmax_ref(x, y); });
+    if (threadIdx.x == 0) {
+        float t = m;
+        t *= flags[b * kFlagStride + 1] == 0 ? cutA : cut;
+        scal[b * kScalStride] = t;
     }
 }
 
@@ -280,18 +360,39 @@ int blur_grid(int W, int H)
     return ((W + kBlurTX - 1) / kBlurTX) * ((H + kBlurTY - 1) / kBlurTY);
 }
 
+template <int R>
+static int launch_blur_r(const BlurArgs &a, hipStream_t s)
+{
+    constexpr int AWP = (kBlurTX + 2 * R + 3) & ~3, AH = kBlurTY + 2 * R;
+    constexpr size_t lds = sizeof(float) * (size_t) AWP * AH;
+    if constexpr (lds > 64 * 1024) {
+        static const hipError_t attr = hipFuncSetAttribute(
+            (const void *) k_blur2d<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+        if (attr != hipSuccess) return from_hip(attr);
+    }
+    const dim3 grid((a.W + kBlurTX - 1) / kBlurTX, (a.H + kBlurTY - 1) / kBlurTY,
+                    a.batch > 1 ? a.batch : 1);
+    hipLaunchKernelGGL(k_blur2d<R>, grid, dim3(256), lds, s, a);
+    return from_hip(hipGetLastError());
+}
+
+template <int... Rs>
+static int launch_blur_dispatch(int r, const BlurArgs &a, hipStream_t s,
+                                std::integer_sequence<int, Rs...>)
+{
+    int st = IMRESH_ERR_INVALID;
+    ((r == Rs ? (st = launch_blur_r<Rs>(a, s), true) : false) || ...);
+    return st;
+}
+
+// one tap radius for both directions: rh == rv, or the identity direction's
+// radius is 0 (the halo of the copied direction is then still staged)
 int launch_blur(const BlurArgs &a, hipStream_t s)
 {
-    const dim3 grid((a.W + kBlurTX - 1) / kBlurTX, (a.H + kBlurTY - 1) / kBlurTY);
-    const int AW = kBlurTX + 2 * a.rh, AH = kBlurTY + 2 * a.rv;
-    const size_t lds = sizeof(float) * (128 + (size_t) AH * AW + (size_t) AH * kBlurTX);
-    static const hipError_t attr =
-        hipFuncSetAttribute((const void *) k_blur2d, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int) (sizeof(float) * (128 + (kBlurTY + 62) * (kBlurTX + 62) +
-                                                    (kBlurTY + 62) * kBlurTX)));
-    if (attr != hipSuccess) return from_hip(attr);
-    hipLaunchKernelGGL(k_blur2d, grid, dim3(256), lds, s, a);
-    return from_hip(hipGetLastError());
+    const int r = a.h_identity ? a.rv : a.v_identity ? a.rh : a.rh;
+    if (!a.h_identity && !a.v_identity && a.rh != a.rv) return IMRESH_ERR_INVALID;
+    if (r < 0 || r > kMaxBlurRadius || a.W < 1 || a.H < 1) return IMRESH_ERR_INVALID;
+    return launch_blur_dispatch(r, a, s, std::make_integer_sequence<int, kMaxBlurRadius + 1>{});
 }
 
 int launch_blur_h_inplace(float *d, int W, int H, const float *w, int r, hipStream_t s)
@@ -316,6 +417,89 @@ int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float t
 {
     hipLaunchKernelGGL(k_err_finalize, dim3(1), dim3(1024), 0, s, perr, pcnt, np, target, res,
                        done, cycles);
+    return from_hip(hipGetLastError());
+}
+
+// batched error finalizer: workgroup b sums the np row-pass partials of
+// image b (in the order of k_err_finalize, so a batch reports the same
+// error as the single-image path), writes {error, total, masked} after its
+// threshold slot, counts the cycle and raises its done flag below the target
+__global__ void __launch_bounds__(1024) k_err_finalize_batch(const float *perr, const unsigned *pcnt,
+                                                           int np, float target, float *scal,
+                                                           int *flags)
+{
+    const int b = blockIdx.x;
+    if (flags[b * kFlagStride]) return;
+    __shared__ float red_f[1024];
+    __shared__ unsigned red_u[1024];
+    float sum = 0.f;
+    unsigned c = 0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        sum += perr[(size_t) b * np + i];
+        c += pcnt[(size_t) b * np + i];
+    }
+    sum = block_reduce(sum, red_f, [](float x, float y) { return x + y; });
+    c = block_reduce(c, red_u, [](unsigned x, unsigned y) { return x + y; });
+    if (threadIdx.x == 0) {
+        const float nm = (float) c;
+        const float err = sqrtf(sum) / nm;
+        float *res = scal + b * kScalStride + 1;
+        res[0] = err;
+        res[1] = sum;
+        res[2] = nm;
+        flags[b * kFlagStride + 1] += 1;
+        if (target > 0 && err < target) flags[b * kFlagStride] = 1;
+    }
+}
+
+int launch_max_finalize_batch(const float *pmax, int np, float cutA, float cut, float *scal,
+                              int *flags, int batch, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_max_finalize_batch, dim3(batch), dim3(1024), 0, s, pmax, np, cutA, cut, scal,
+                       flags);
+    return from_hip(hipGetLastError());
+}
+
+int launch_err_finalize_batch(const float *perr, const unsigned *pcnt, int np, float target,
+                              float *scal, int *flags, int batch, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_err_finalize_batch, dim3(batch), dim3(1024), 0, s, perr, pcnt, np, target,
+                       scal, flags);
+    return from_hip(hipGetLastError());
+}
+
+// batch staging: image b of a pointer table <-> plane b of a batch buffer
+__global__ void __launch_bounds__(256) k_gather_images(float *dst, const float *const *src, size_t n)
+{
+    const int b = blockIdx.y;
+    const float *p = src[b];
+    float *q = dst + (size_t) b * n;
+    for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n;
+         i += (size_t) gridDim.x * blockDim.x)
+        q[i] = p[i];
+}
+
+__global__ void __launch_bounds__(256) k_scatter_real(float *const *dst, const float2 *X, size_t n)
+{
+    const int b = blockIdx.y;
+    float *q = dst[b];
+    const float2 *p = X + (size_t) b * n;
+    for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n;
+         i += (size_t) gridDim.x * blockDim.x)
+        q[i] = p[i].x;
+}
+
+int launch_gather_images(float *dst, const float *const *src, size_t n, int batch, hipStream_t s)
+{
+    const dim3 grid((unsigned) std::min<size_t>((n + 255) / 256, 1024), batch);
+    hipLaunchKernelGGL(k_gather_images, grid, dim3(256), 0, s, dst, src, n);
+    return from_hip(hipGetLastError());
+}
+
+int launch_scatter_real(float *const *dst, const float2 *X, size_t n, int batch, hipStream_t s)
+{
+    const dim3 grid((unsigned) std::min<size_t>((n + 255) / 256, 1024), batch);
+    hipLaunchKernelGGL(k_scatter_real, grid, dim3(256), 0, s, dst, X, n);
     return from_hip(hipGetLastError());
 }
 

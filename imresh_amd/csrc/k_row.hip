@@ -28,11 +28,15 @@ namespace {
 
 enum RowKernel { RK_FWD, RK_INV, RK_INVOUT, RK_FIRST, RK_MID };
 
-// NL rows per workgroup
-template <class P, int NL, int K>
+// NL rows per workgroup; PF (RK_MID only): gPrev and the mask are read right
+// after the row gather instead of at the junction
+template <class P, int NL, int K, bool PF = false>
 __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
 {
-    if (a.done != nullptr && *a.done) return;
+    // image b of a batch (grid.y): its planes start b * istride elements in,
+    // its scalars at b * kScalStride / kFlagStride
+    const int b = blockIdx.y;
+    if (a.done != nullptr && a.done[b * kFlagStride]) return;
     extern __shared__ cf smem[];
     constexpr int n = P::n, T = P::T;
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
@@ -40,7 +44,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     const bool valid = row < a.H;
     // workgroup base (uniform) + 32-bit per-lane offsets; rows past the end
     // read row 0 of the workgroup and never store
-    const size_t wb = (size_t) blockIdx.x * NL * n;
+    const size_t wb = (size_t) b * a.istride + (size_t) blockIdx.x * NL * n;
     const int lo = valid ? l * n : 0;
     const LdsC<1> L{smem, l * n};
     const cf *tw = (const cf *) a.tw;
@@ -83,13 +87,13 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             err = block_reduce(err, red_f, [](float x, float y) { return x + y; });
             cnt = block_reduce(cnt, red_u, [](unsigned x, unsigned y) { return x + y; });
             if (threadIdx.x == 0) {
-                a.perr[blockIdx.x] = err;
-                a.pcnt[blockIdx.x] = cnt;
+                a.perr[b * gridDim.x + blockIdx.x] = err;
+                a.pcnt[b * gridDim.x + blockIdx.x] = cnt;
             }
         }
     } else if constexpr (K == RK_FIRST) {
         const bool cut = (a.flags & kRowCut) != 0;
-        const float thr = cut ? *a.thr : 0.f;
+        const float thr = cut ? a.thr[b * kScalStride] : 0.f;
         cf *G = (cf *) a.G + wb;
         uint8_t *mask = a.mask + wb;
         const float *B2 = a.B2 + wb;
@@ -112,20 +116,30 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         cf *G = (cf *) a.G + wb;
         const uint8_t *mask = a.mask + wb;
         const float beta = a.beta;
-        auto mid = [=](int e, cf g) {
-            const cf gp = hio_constraint(at32(G, lo + e), g, at32(mask, lo + e) == 1, beta);
+        struct GM {
+            cf g;
+            bool m;
+        };
+        auto pre = [=](int e) {
+            if constexpr (PF) return GM{at32(G, lo + e), at32(mask, lo + e) == 1};
+            else return NoPre{};
+        };
+        auto mid = [=](int e, cf g, auto p) {
+            cf gp;
+            if constexpr (PF) gp = hio_constraint(p.g, g, p.m, beta);
+            else gp = hio_constraint(at32(G, lo + e), g, at32(mask, lo + e) == 1, beta);
             if (valid) at32(G, lo + e) = gp;
             return gp;
         };
-        line_pair<P, +1, -1>(v, tw, t, L, ldX, mid, stX);
+        line_pair<P, +1, -1>(v, tw, t, L, ldX, pre, mid, stX);
     }
 }
 
-template <class P, int NL>
+template <class P, int NL, bool PF = false>
 int launch_variant(int mode, RowArgs a, hipStream_t s)
 {
     static_assert(NL * P::T <= 1024, "row workgroup too large");
-    const dim3 grid((a.H + NL - 1) / NL), block(NL * P::T);
+    const dim3 grid((a.H + NL - 1) / NL, a.batch > 1 ? a.batch : 1), block(NL * P::T);
     constexpr size_t lds = sizeof(cf) * (size_t) lds_elems(NL * P::n);
     static_assert(lds <= kLdsBudget, "row tile exceeds the LDS budget");
     if constexpr (lds > 64 * 1024) {
@@ -133,7 +147,8 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
             hipError_t e = hipSuccess;
             for (const void *f : {(const void *) k_row<P, NL, RK_FWD>, (const void *) k_row<P, NL, RK_INV>,
                                   (const void *) k_row<P, NL, RK_INVOUT>,
-                                  (const void *) k_row<P, NL, RK_FIRST>, (const void *) k_row<P, NL, RK_MID>})
+                                  (const void *) k_row<P, NL, RK_FIRST>,
+                                  (const void *) k_row<P, NL, RK_MID, PF>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -163,45 +178,18 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         a.flags = 0;
         hipLaunchKernelGGL((k_row<P, NL, RK_FIRST>), grid, block, lds, s, a);
         break;
-    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID>), grid, block, lds, s, a); break;
+    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID, PF>), grid, block, lds, s, a); break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());
 }
 
-// Row variants of a length: 0 = E 16 with ~256-thread groups; for long rows
-// also E 32 (half the threads per row) and E 16 with twice the rows.
+// Row variants of a length (index: workgroup shape, then +count/2 for the
+// prefetching RK_MID): shapes 0 = E 16 with ~256-thread groups; for long
+// rows also 1 = E 32 (half the threads per row) and 2 = E 16 with twice the
+// rows.
 template <int N, int... Rs>
-int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
-{
-    using P16 = PlanE<16, N, Rs...>;
-    if constexpr (N >= 1024) {
-        using P32 = PlanE<32, N, Rs...>;
-        if (var == 1) return launch_variant<P32, P32::L>(mode, a, s);
-        if constexpr (2 * P16::L * P16::T <= 1024 &&
-                      sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget)
-            if (var == 2) return launch_variant<P16, 2 * P16::L>(mode, a, s);
-    }
-    return launch_variant<P16, P16::L>(mode, a, s);
-}
-
-// rows per workgroup of a variant (the host needs it for the partial sums)
-template <int N, int... Rs>
-int row_variant_rows(int var)
-{
-    using P16 = PlanE<16, N, Rs...>;
-    if constexpr (N >= 1024) {
-        using P32 = PlanE<32, N, Rs...>;
-        if (var == 1) return P32::L;
-        if constexpr (2 * P16::L * P16::T <= 1024 &&
-                      sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget)
-            if (var == 2) return 2 * P16::L;
-    }
-    return P16::L;
-}
-
-template <int N, int... Rs>
-constexpr int row_variant_count()
+constexpr int row_shape_count()
 {
     if constexpr (N >= 1024) {
         using P16 = PlanE<16, N, Rs...>;
@@ -209,6 +197,43 @@ constexpr int row_variant_count()
                 sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 3 : 2;
     }
     return 1;
+}
+
+template <int N, int... Rs>
+constexpr int row_variant_count() { return 2 * row_shape_count<N, Rs...>(); }
+
+template <int N, int... Rs>
+int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
+{
+    using P16 = PlanE<16, N, Rs...>;
+    constexpr int ns = row_shape_count<N, Rs...>();
+    const bool pf = var >= ns;
+    const int shape = pf ? var - ns : var;
+    if constexpr (N >= 1024) {
+        using P32 = PlanE<32, N, Rs...>;
+        if (shape == 1)
+            return pf ? launch_variant<P32, P32::L, true>(mode, a, s) : launch_variant<P32, P32::L>(mode, a, s);
+        if constexpr (ns == 3)
+            if (shape == 2)
+                return pf ? launch_variant<P16, 2 * P16::L, true>(mode, a, s)
+                          : launch_variant<P16, 2 * P16::L>(mode, a, s);
+    }
+    return pf ? launch_variant<P16, P16::L, true>(mode, a, s) : launch_variant<P16, P16::L>(mode, a, s);
+}
+
+// rows per workgroup of a variant (the host needs it for the partial sums)
+template <int N, int... Rs>
+int row_variant_rows(int var)
+{
+    using P16 = PlanE<16, N, Rs...>;
+    constexpr int ns = row_shape_count<N, Rs...>();
+    const int shape = var >= ns ? var - ns : var;
+    if constexpr (N >= 1024) {
+        using P32 = PlanE<32, N, Rs...>;
+        if (shape == 1) return P32::L;
+        if (shape == 2) return 2 * P16::L;
+    }
+    return P16::L;
 }
 
 template <int G, int N, int... Rs>

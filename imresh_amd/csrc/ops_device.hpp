@@ -64,12 +64,53 @@ __device__ __forceinline__ cf modulus_project_fused(cf z, float m)
     return modulus_project(z, m);
 }
 
-// deterministic workgroup reduction through LDS (any blockDim <= 1024):
-// a fixed pairwise tree, so results do not depend on scheduling
+// one wave64 lane move of a 32-bit value through DPP (CTRL: the DPP control
+// word; every source lane used below lies inside the same row of 16 lanes)
+template <int CTRL, class T>
+__device__ __forceinline__ T dpp_move(T v)
+{
+    static_assert(sizeof(T) == 4, "32-bit lanes");
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+template <class T>
+__device__ __forceinline__ T read_lane(T v, int lane)
+{
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// wave64 reduction for a wave whose 64 lanes are all active: DPP butterflies
+// inside each row of 16 lanes (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror), then the four row results combined in a fixed
+// order.  Every lane returns the same, scheduling-independent value.
+template <class T, class Op>
+__device__ __forceinline__ T wave_reduce(T x, Op op)
+{
+    x = op(x, dpp_move<0xB1>(x));
+    x = op(x, dpp_move<0x4E>(x));
+    x = op(x, dpp_move<0x141>(x));
+    x = op(x, dpp_move<0x140>(x));
+    return op(op(read_lane(x, 0), read_lane(x, 16)), op(read_lane(x, 32), read_lane(x, 48)));
+}
+
+// deterministic workgroup reduction (any blockDim <= 1024; red holds at least
+// blockDim entries): whole waves reduce with wave_reduce and combine their
+// results in wave order; a block with a partial wave uses a fixed pairwise
+// LDS tree.  Results do not depend on scheduling.
 template <class T, class Op>
 __device__ __forceinline__ T block_reduce(T x, T *red, Op op)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
+    if ((nt & 63) == 0) {
+        x = wave_reduce(x, op);
+        __syncthreads();
+        if ((tid & 63) == 0) red[tid >> 6] = x;
+        __syncthreads();
+        T r = red[0];
+        for (int i = 1; i < (nt >> 6); ++i) r = op(r, red[i]);
+        return r;
+    }
     __syncthreads();
     red[tid] = x;
     __syncthreads();

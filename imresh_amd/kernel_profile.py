@@ -3,9 +3,9 @@
     python -m imresh_amd.kernel_profile --size 2048 --reps 8
 
 launches, in this order on the default stream:
-  1. a plain 32 MiB device-to-device copy (hipMemcpyAsync), the calibration
-     kernel of known traffic (32 MiB read + 32 MiB written) for FETCH_SIZE /
-     WRITE_SIZE;
+  1. a plain 128 MiB device-to-device copy (hipMemcpyAsync, 4x the aggregate
+     L2), the calibration kernel of known traffic (128 MiB read + 128 MiB
+     written) for FETCH_SIZE / WRITE_SIZE;
   2. imresh_profile_kernels: each fused kernel of one HIO iteration at
      size x size back to back (row pass, column pass, blur, closing row pass),
      and prints their HIP-event durations.
@@ -24,7 +24,7 @@ def main() -> int:
     ap.add_argument("--reps", type=int, default=8)
     a = ap.parse_args()
     L = im.lib()
-    nbytes = 32 << 20
+    nbytes = 128 << 20
     src, dst = im.DeviceBuffer(nbytes), im.DeviceBuffer(nbytes)
     for _ in range(3):
         im.check(L.imresh_memcpy_d2d(dst.ptr, src.ptr, nbytes, None))

@@ -104,6 +104,29 @@ int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
 int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
                              const imresh_params *params, void *stream, imresh_result *result);
 
+/* ---- batched reconstruction (no reference counterpart: the reference runs
+ * one image per cudaShrinkWrap call and overlaps calls only through its task
+ * queue, taskQueue.cu:91-193) -------------------------------------------- */
+typedef struct imresh_image {
+    float   *data;            /* width*height floats: |F| in, Re(g') out     */
+    uint32_t width;
+    uint32_t height;
+} imresh_image;
+
+/* Reconstructs `count` independent images of any supported sizes together
+ * with one parameter set (the reference's per-call defaults).  Images of one
+ * size are stacked and run through batched kernels; each cycle of all sizes
+ * is one HIP graph replay.  Each image's result equals
+ * imresh_shrinkwrap(images[i], params) run with the same kernel variants.
+ * images[i].data are host buffers; results (may be NULL) receives one
+ * imresh_result per image.  Synchronous. */
+int imresh_shrinkwrap_batch(const imresh_image *images, uint32_t count,
+                            const imresh_params *params, void *stream, imresh_result *results);
+/* Same with images[i].data device buffers (inputs resident in HBM). */
+int imresh_shrinkwrap_batch_device(const imresh_image *images, uint32_t count,
+                                   const imresh_params *params, void *stream,
+                                   imresh_result *results);
+
 /* HIO inner loop from a given device state, n_iter iterations
  * (cudaShrinkWrap.cu:311-365 / shrinkWrap.cpp:260-319): gprime (complex, g'),
  * gprev (complex), mask (uint8, 1 = outside support), modulus (float).
