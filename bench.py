@@ -88,12 +88,184 @@ def cpu_baseline(M: np.ndarray, budget_s: float = 12.0, max_iter: int = 20) -> d
                       f"{dt:.1f} s"}
 
 
+def _gen_batch_input(args):
+    i, n = args
+    shapes, central = syn.batch_shapes(n)
+    return syn.workload(n, n, n_shapes=shapes, central=central, photons=1e7,
+                        index=20_000_000 + i)[1]
+
+
+def generate_inputs(batch_count: int, serve_count: int):
+    """All synthetic inputs, generated in worker processes BEFORE this
+    process touches the GPU (workers are forked; a process that has
+    initialized the GPU must not fork helpers)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    sizes = syn.batch_sizes(batch_count) if batch_count else []
+    trace = syn.serve_trace(serve_count) if serve_count else []
+    log(f"[bench] generating {len(sizes)} batch and {len(trace)} serving inputs")
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 4),
+                             mp_context=mp.get_context("fork")) as ex:
+        bmods = list(ex.map(_gen_batch_input, list(enumerate(sizes)), chunksize=8))
+        smods = list(ex.map(_gen_serve_input, [(i, n) for i, (_, n, _) in enumerate(trace)],
+                            chunksize=4))
+    return (sizes, bmods), (trace, smods)
+
+
+def batch_bench(L, sizes, mods, reps: int = 2, plain_streams: int = 3) -> dict:
+    """Batched workload: `count` images of uneven sizes (synthetic.config_batch),
+    20 x 20 each, inputs resident in HBM.  Times one batched call
+    (imresh_shrinkwrap_batch_device) against the plain loop: one
+    imresh_shrinkwrap_device per image, round-robin over `plain_streams`
+    streams (the reference task queue's 3 streams per device), no host
+    synchronization until the end.  Both outputs are compared bitwise."""
+    count = len(sizes)
+    srcs = [im.DeviceBuffer.from_array(m) for m in mods]
+    outs_b = [im.DeviceBuffer(m.nbytes) for m in mods]
+    outs_p = [im.DeviceBuffer(m.nbytes) for m in mods]
+    p = im.default_params(n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
+
+    def restore(outs):
+        for a, b, m in zip(srcs, outs, mods):
+            im.check(L.imresh_memcpy_d2d(b.ptr, a.ptr, m.nbytes, None), "d2d")
+        im.sync()
+
+    images = [(o.ptr, n, n) for o, n in zip(outs_b, sizes)]
+    streams = [im.stream_create() for _ in range(plain_streams)]
+
+    def plain():
+        for i, (o, n) in enumerate(zip(outs_p, sizes)):
+            im.check(L.imresh_shrinkwrap_device(o.ptr, n, n, p, streams[i % plain_streams], None),
+                     "shrinkwrap")
+        for st in streams:
+            im.sync(st)
+
+    log("[bench] batch warmup (autotunes every size)")
+    restore(outs_b)
+    im.shrink_wrap_batch_device(images, p, want_results=False)
+    restore(outs_p)
+    plain()
+    tb, tp = [], []
+    for _ in range(reps):
+        restore(outs_b)
+        t0 = time.perf_counter()
+        im.shrink_wrap_batch_device(images, p, want_results=False)
+        tb.append(time.perf_counter() - t0)
+        restore(outs_p)
+        t0 = time.perf_counter()
+        plain()
+        tp.append(time.perf_counter() - t0)
+    same = all(np.array_equal(a.download(np.float32, m.shape), b.download(np.float32, m.shape))
+               for a, b, m in zip(outs_b, outs_p, mods))
+    for st in streams:
+        im.stream_destroy(st)
+    elems = sum(n * n for n in sizes)
+    rb, rp = count / min(tb), count / min(tp)
+    log(f"[bench] batched {rb:.1f} recon/s, plain loop {rp:.1f} recon/s, bitwise equal {same}")
+    return {"images": count, "sizes": "uniform over " + ",".join(map(str, syn.BATCH_SIZES)),
+            "elements": elems, "photons": 1e7, "batched_recon_per_s": round(rb, 2),
+            "plain_loop_recon_per_s": round(rp, 2), "plain_loop_streams": plain_streams,
+            "speedup": round(rb / rp, 3), "batched_s": round(min(tb), 4), "plain_s": round(min(tp), 4),
+            "batched_alg_GBps_80B": round(80 * elems * N_CYCLES * N_HIO / min(tb) / 1e9, 1),
+            "outputs_bitwise_equal": bool(same)}
+
+
+def _gen_serve_input(args):
+    i, n = args
+    return syn.fast_modulus(n, i)
+
+
+def _percentiles(lat):
+    a = np.sort(np.asarray(lat))
+    return {"p50_ms": round(float(np.percentile(a, 50)), 2), "p99_ms": round(float(np.percentile(a, 99)), 2),
+            "p99.9_ms": round(float(np.percentile(a, 99.9)), 2), "max_ms": round(float(a[-1]), 2)}
+
+
+def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200) -> dict:
+    """Serving workload: `count` requests of synthetic.serve_trace (Poisson
+    arrivals at `rate`/s, sizes log-uniform in [128, 2048], 5/10/20 cycles x
+    20 HIO), host buffers as the reference API takes them.  The same trace is
+    replayed open-loop (each request submitted at its arrival time) through
+    the continuous-batching server and through the plain task queue (3
+    streams, one reconstruction per slot, addTask blocking when all slots are
+    busy).  Latency = completion - scheduled arrival."""
+    import threading
+
+    count = len(trace)
+    params = [im.default_params(n_cycles=c, n_hio_cycles=N_HIO, target_error=FLT_MIN) for _, _, c in trace]
+    iters = sum(c * N_HIO for _, _, c in trace)
+
+    def replay(submit, wait):
+        ios = [m.copy() for m in mods]
+        t0 = time.perf_counter()
+        for k, ((a, n, c), io) in enumerate(zip(trace, ios)):
+            dt = t0 + a - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            submit(k, io)
+        wait()
+        return t0, ios
+
+    # continuous-batching server
+    done_t = {}
+    with im.Server(hbm_budget_bytes=budget_gib << 30) as srv:
+        ids = {}
+
+        def sub(k, io):
+            ids[srv.submit(io, params[k])] = k
+
+        t0, ios_s = replay(sub, srv.drain)
+        comp = srv.completions(count + 16)
+        peak = srv.stats().peak_bytes
+    # completion wall time per request from submit-relative latency
+    sub_lat = {ids[c[0]]: c[3] for c in comp}
+    bad = sum(1 for c in comp if c[1] != 0)
+    # latency measured from the scheduled arrival: the submit of request k
+    # happened at or after its arrival, latency_ms counts from the submit
+    lat_s = [sub_lat[k] for k in range(count)]
+    span_s = max(trace[k][0] + sub_lat[k] / 1e3 for k in range(count))
+
+    # plain task queue
+    lock = threading.Lock()
+    t_done = [0.0] * count
+    im.task_queue_init(0)
+    tq0 = [0.0]
+
+    def sub_q(k, io):
+        def cb(arr, w, h, f, k=k):
+            with lock:
+                t_done[k] = time.perf_counter()
+        _, n, c = trace[k]
+        im.add_task(io, cb, "", n_cycles=c, n_hio_cycles=N_HIO, target_error=FLT_MIN,
+                    cutoff_autocorel=0.04)
+
+    tq0[0], ios_q = replay(sub_q, lambda: None)
+    im.task_queue_deinit()
+    lat_q = [(t_done[k] - (tq0[0] + trace[k][0])) * 1e3 for k in range(count)]
+    span_q = max(t_done) - tq0[0]
+    same = all(np.array_equal(a, b) for a, b in zip(ios_s, ios_q))
+    out = {"requests": count, "arrival_rate_per_s": rate, "sizes": "round(exp(U[ln128, ln2048]))",
+           "cycles": "5/10/20 x 20 HIO", "hbm_budget_gib": budget_gib, "peak_state_gib": round(peak / 2**30, 3),
+           "server": {"recon_per_s": round(count / span_s, 2),
+                      "hio_iters_per_s": round(iters / span_s, 1), **_percentiles(lat_s), "errors": bad},
+           "task_queue": {"recon_per_s": round(count / span_q, 2),
+                          "hio_iters_per_s": round(iters / span_q, 1), **_percentiles(lat_q)},
+           "outputs_bitwise_equal": bool(same)}
+    out["speedup_recon_per_s"] = round(out["server"]["recon_per_s"] / out["task_queue"]["recon_per_s"], 3)
+    log(f"[bench] serving: server {out['server']}, task queue {out['task_queue']}, equal {same}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=300)
+    ap.add_argument("--batch-count", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): independent images shard "
@@ -102,6 +274,8 @@ def main() -> int:
 
     log("[bench] generating the 2048x2048 workload")
     obj, M = syn.config_2048(0)
+    (bsizes, bmods), (trace, smods) = generate_inputs(0 if args.no_batch else args.batch_count,
+                                                      0 if args.no_serve else args.serve_count)
     d_src = im.DeviceBuffer.from_array(M)
     d_io = im.DeviceBuffer(M.nbytes)
     L = im.lib()
@@ -156,6 +330,14 @@ def main() -> int:
     # HIO iteration (76 + 4-byte mask); the fused path moves 53 B/element
     roofline["iteration_alg_GBps_80B"] = round(80 * n * value / 1e9, 1)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_bench(L, bsizes, bmods)
+
+    serve = None
+    if not args.no_serve:
+        serve = serve_bench(L, trace, smods)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
@@ -170,7 +352,8 @@ def main() -> int:
                                "sigma0 3.0, cut 0.2/0.04, 64 shapes in central 1024^2",
                    "iterations_per_step": N_CYCLES * N_HIO, "width": W, "height": H,
                    "row_variant": rv.value, "col_variant": cv.value},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batch,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
     return 0

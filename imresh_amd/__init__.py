@@ -52,6 +52,20 @@ class Image(ctypes.Structure):
     _fields_ = [("data", c_void_p), ("width", c_uint32), ("height", c_uint32)]
 
 
+class ServerCompletion(ctypes.Structure):
+    """imresh_server_completion."""
+    _fields_ = [("id", c_uint64), ("status", ctypes.c_int32), ("result", Result),
+                ("latency_ms", c_double)]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("queued", c_uint64),
+                ("active", c_uint64), ("bytes_in_use", c_uint64), ("peak_bytes", c_uint64),
+                ("budget", c_uint64)]
+
+
+SERVER_CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, POINTER(ServerCompletion), POINTER(c_float))
+
 TASK_CALLBACK = ctypes.CFUNCTYPE(None, c_void_p, POINTER(c_float), c_uint32, c_uint32, c_int)
 
 _LIB = None
@@ -69,6 +83,8 @@ _SIGS = {
     "imresh_memcpy_d2h": (c_int, [_P, _P, c_uint64, _P]),
     "imresh_memcpy_d2d": (c_int, [_P, _P, c_uint64, _P]),
     "imresh_stream_sync": (c_int, [_P]),
+    "imresh_stream_create": (c_int, [POINTER(c_void_p)]),
+    "imresh_stream_destroy": (c_int, [_P]),
     "imresh_shrinkwrap": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), _P, POINTER(Result)]),
     "imresh_shrinkwrap_device": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), _P,
                                          POINTER(Result)]),
@@ -76,6 +92,13 @@ _SIGS = {
                                         POINTER(Result)]),
     "imresh_shrinkwrap_batch_device": (c_int, [POINTER(Image), c_uint32, POINTER(Params), _P,
                                                POINTER(Result)]),
+    "imresh_server_create": (c_int, [c_uint64, c_uint32, c_uint32, POINTER(c_void_p)]),
+    "imresh_server_submit": (c_int, [_P, _P, c_uint32, c_uint32, POINTER(Params), c_int,
+                                     SERVER_CALLBACK, _P, POINTER(c_uint64)]),
+    "imresh_server_poll": (c_int, [_P, POINTER(ServerCompletion), c_uint32, POINTER(c_uint32)]),
+    "imresh_server_drain": (c_int, [_P]),
+    "imresh_server_stats": (c_int, [_P, POINTER(ServerInfo)]),
+    "imresh_server_destroy": (c_int, [_P]),
     "imresh_hio_iterations": (c_int, [_P, _P, _P, _P, c_uint32, c_uint32, c_float, c_uint32, _P]),
     "imresh_copy_to_real_part": (c_int, [_P, _P, c_uint32, _P]),
     "imresh_copy_from_real_part": (c_int, [_P, _P, c_uint32, _P]),
@@ -197,6 +220,16 @@ def sync(stream=None) -> None:
     check(_lib().imresh_stream_sync(stream), "imresh_stream_sync")
 
 
+def stream_create() -> int:
+    s = c_void_p()
+    check(_lib().imresh_stream_create(ctypes.byref(s)), "imresh_stream_create")
+    return s.value
+
+
+def stream_destroy(stream: int) -> None:
+    check(_lib().imresh_stream_destroy(stream), "imresh_stream_destroy")
+
+
 # ------------------------------------------------------------ reference API
 def shrink_wrap(io: np.ndarray, n_cycles: int = 20, target_error: float = 1e-5,
                 hio_beta: float = 0.9, cutoff_autocorel: float = 0.04, cutoff: float = 0.20,
@@ -253,6 +286,66 @@ def shrink_wrap_batch_device(images: list, params: Params | None = None, stream=
                                                 res if want_results else None),
           "imresh_shrinkwrap_batch_device")
     return [res[i] for i in range(len(images))] if want_results else None
+
+
+class Server:
+    """Continuous-batching reconstruction server (imresh_server_*).  submit()
+    takes a float32 H x W host array (kept alive until completion) or a
+    (device_ptr, width, height) tuple; completions() returns the finished
+    requests as (id, status, Result, latency_ms)."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, max_active: int = 0, lanes: int = 0):
+        h = c_void_p()
+        check(_lib().imresh_server_create(hbm_budget_bytes, max_active, lanes, ctypes.byref(h)),
+              "imresh_server_create")
+        self._h = h.value
+        self._keep: dict = {}
+
+    def submit(self, io, params: Params | None = None) -> int:
+        p = params if params is not None else default_params()
+        rid = c_uint64()
+        if isinstance(io, tuple):
+            ptr, w, h = io
+            dev = 1
+        else:
+            if io.dtype != np.float32 or not io.flags.c_contiguous or io.ndim != 2:
+                raise ValueError("io must be a C-contiguous float32 H x W array")
+            ptr, (h, w), dev = io.ctypes.data, io.shape, 0
+        check(_lib().imresh_server_submit(self._h, ptr, w, h, ctypes.byref(p), dev,
+                                          SERVER_CALLBACK(), None, ctypes.byref(rid)),
+              "imresh_server_submit")
+        self._keep[rid.value] = io
+        return rid.value
+
+    def completions(self, capacity: int = 4096) -> list:
+        buf = (ServerCompletion * capacity)()
+        n = c_uint32()
+        check(_lib().imresh_server_poll(self._h, buf, capacity, ctypes.byref(n)), "imresh_server_poll")
+        out = []
+        for i in range(n.value):
+            c = buf[i]
+            self._keep.pop(c.id, None)
+            out.append((c.id, c.status, c.result, c.latency_ms))
+        return out
+
+    def drain(self) -> None:
+        check(_lib().imresh_server_drain(self._h), "imresh_server_drain")
+
+    def stats(self) -> ServerInfo:
+        info = ServerInfo()
+        check(_lib().imresh_server_stats(self._h, ctypes.byref(info)), "imresh_server_stats")
+        return info
+
+    def close(self) -> None:
+        if self._h:
+            check(_lib().imresh_server_destroy(self._h), "imresh_server_destroy")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 _TASK_REFS: dict = {}

@@ -36,31 +36,10 @@ namespace {
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-struct Workspace {
-    int device = 0;
-    int W = 0, H = 0;
-    size_t n = 0;
-    void *base = nullptr;
-    float2 *X = nullptr, *G = nullptr;
-    float *Bn = nullptr, *B2 = nullptr, *M = nullptr;
-    uint8_t *mask = nullptr;
-    float *pmax = nullptr;
-    float *perr = nullptr;
-    unsigned *pcnt = nullptr;
-    float *scal = nullptr;      // [0] threshold, [1..3] error, total, nmasked
-    int *flags = nullptr;       // [0] done, [1] cycles
-    float *weights = nullptr;   // [cycles][64]
-    unsigned wcap = 0;
-    std::vector<int> taps;
-    float w_sigma0 = -1.f, w_dsigma = -1.f;
-    unsigned w_ncycles = 0;
-    float *host_stage = nullptr;  // pinned, 64 floats + readback
-    hipEvent_t ready = nullptr;   // recorded when the last user's work is queued
-    bool pending = false;
-};
 
-std::mutex g_ws_mtx;
-std::map<std::tuple<int, int, int>, std::vector<Workspace *>> g_ws_free;
+}  // namespace
+
+namespace imr {
 
 int ws_alloc(int device, int W, int H, Workspace **out)
 {
@@ -84,6 +63,7 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     const size_t oPcnt = off; off = align256(off + 4 * (size_t) nrow);
     const size_t oScal = off; off = align256(off + 64);
     const size_t oFlags = off; off = align256(off + 64);
+    w->bytes = off;
     hipError_t e = hipMalloc(&w->base, off);
     if (e != hipSuccess) {
         delete w;
@@ -113,20 +93,67 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     return IMRESH_OK;
 }
 
+void ws_release(Workspace *w)
+{
+    if (w == nullptr) return;
+    if (w->base) hipFree(w->base);
+    if (w->weights) hipFree(w->weights);
+    if (w->host_stage) hipHostFree(w->host_stage);
+    if (w->ready) hipEventDestroy(w->ready);
+    delete w;
+}
+
+size_t ws_bytes(int W, int H)
+{
+    const size_t n = (size_t) W * H;
+    return 29 * n + 4 * (size_t) (blur_grid(W, H) + 2 * H) + 8 * 256;
+}
+
+}  // namespace imr
+
+namespace {
+
+std::mutex g_ws_mtx;
+std::map<std::tuple<int, int, int>, std::vector<Workspace *>> g_ws_free;
+
 // Workspaces are reused in stream order: a checked-out workspace makes the
 // new stream wait for the previous user's work (no host synchronization).
+// A size keeps up to kMaxWorkspaces of them so reconstructions of one size
+// on different streams run concurrently: an idle one (its last work done) is
+// preferred, a new one is allocated while under the cap, and only then does
+// the new stream queue behind a busy one.
+constexpr int kMaxWorkspaces = 4;
+std::map<std::tuple<int, int, int>, int> g_ws_count;
+
 int ws_checkout(int device, int W, int H, hipStream_t s, Workspace **out)
 {
     Workspace *w = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_ws_mtx);
         auto &v = g_ws_free[{device, W, H}];
-        if (!v.empty()) {
-            w = v.back();
-            v.pop_back();
+        for (size_t i = 0; i < v.size() && w == nullptr; ++i)
+            if (!v[i]->pending || hipEventQuery(v[i]->ready) == hipSuccess) {
+                w = v[i];
+                v.erase(v.begin() + (long) i);
+            }
+        if (w == nullptr) {
+            int &count = g_ws_count[{device, W, H}];
+            if (count >= kMaxWorkspaces && !v.empty()) {
+                w = v.front();
+                v.erase(v.begin());
+            } else {
+                ++count;
+            }
         }
     }
-    if (w == nullptr) return ws_alloc(device, W, H, out);
+    if (w == nullptr) {
+        const int st = ws_alloc(device, W, H, out);
+        if (st != IMRESH_OK) {
+            std::lock_guard<std::mutex> lk(g_ws_mtx);
+            --g_ws_count[{device, W, H}];
+        }
+        return st;
+    }
     if (w->pending) IMR_HIP_CHECK(hipStreamWaitEvent(s, w->ready, 0));
     *out = w;
     return IMRESH_OK;
@@ -166,14 +193,21 @@ int normalize(const imresh_params *p, Params &q)
 
 namespace {
 
-// per-cycle Gaussian weights, uploaded once per parameter set
+}  // namespace
+
+namespace imr {
+
+// per-cycle Gaussian weights, uploaded once per parameter set (a table for
+// more cycles of the same schedule serves fewer cycles as well); the host
+// copy lives in the workspace, so the upload needs no synchronization
 int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
 {
-    if (w->weights && w->w_sigma0 == q.sigma0 && w->w_dsigma == q.dsigma && w->w_ncycles == q.n_cycles)
+    if (w->weights && w->w_sigma0 == q.sigma0 && w->w_dsigma == q.dsigma && w->w_ncycles >= q.n_cycles)
         return IMRESH_OK;
     std::vector<float> sig;
     sigma_schedule(q.sigma0, q.dsigma, q.n_cycles, sig);
-    std::vector<float> host((size_t) std::max(1u, q.n_cycles) * 64, 0.f);
+    std::vector<float> &host = w->host_weights;
+    host.assign((size_t) std::max(1u, q.n_cycles) * 64, 0.f);
     w->taps.assign(q.n_cycles, 0);
     for (unsigned c = 0; c < q.n_cycles; ++c) {
         const int k = gaussian_weights(sig[c], nullptr, 0);
@@ -192,12 +226,15 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
     }
     IMR_HIP_CHECK(hipMemcpyAsync(w->weights, host.data(), sizeof(float) * host.size(),
                                  hipMemcpyHostToDevice, s));
-    IMR_HIP_CHECK(hipStreamSynchronize(s));
     w->w_sigma0 = q.sigma0;
     w->w_dsigma = q.dsigma;
     w->w_ncycles = q.n_cycles;
     return IMRESH_OK;
 }
+
+}  // namespace imr
+
+namespace {
 
 // ---------------------------------------------------------------- autotuner
 // Picks, per device and image size, the fastest row-pass and column-pass
@@ -272,20 +309,22 @@ int autotune(int device, int W, int H, hipStream_t s, Workspace *w)
     return IMRESH_OK;
 }
 
-int run_reconstruction(Workspace *w, float *io_dev, const Params &q, hipStream_t s)
-{
-    IMR_CHECK(autotune(w->device, w->W, w->H, s, w));
-    Plan2D p;
-    IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
-    IMR_CHECK(ensure_weights(w, q, s));
-    IMR_HIP_CHECK(hipMemsetAsync(w->flags, 0, 64, s));
-    IMR_HIP_CHECK(hipMemsetAsync(w->scal, 0, 64, s));
-    const size_t n = w->n;
+}  // namespace
 
+namespace imr {
+
+namespace {
+ColArgs col_args(const Workspace *w, const float *M)
+{
     ColArgs ca{};
     ca.X = w->X;
-    ca.M = io_dev;
+    ca.M = M;
     ca.done = w->flags;
+    return ca;
+}
+
+RowArgs row_args(const Workspace *w, const Params &q)
+{
     RowArgs ra{};
     ra.X = w->X;
     ra.G = w->G;
@@ -297,47 +336,92 @@ int run_reconstruction(Workspace *w, float *io_dev, const Params &q, hipStream_t
     ra.perr = w->perr;
     ra.pcnt = w->pcnt;
     ra.done = w->flags;
+    return ra;
+}
+}  // namespace
 
-    IMR_CHECK(launch_col(COL_INIT, p, ca, s));
-    IMR_CHECK(launch_row(ROW_INIT, p, ra, s));
-    const int nrowwg = row_grid(p);
-    const int nblur = blur_grid(w->W, w->H);
-    for (unsigned c = 0; c < q.n_cycles; ++c) {
-        BlurArgs ba{};
-        ba.in = w->Bn;
-        ba.out = w->B2;
-        ba.wh = ba.wv = w->weights + (size_t) c * 64;
-        ba.rh = ba.rv = (w->taps[c] - 1) / 2;
-        ba.W = w->W;
-        ba.H = w->H;
-        ba.pmax = w->pmax;
-        ba.done = w->flags;
-        IMR_CHECK(launch_blur(ba, s));
-        IMR_CHECK(launch_max_finalize(w->pmax, nblur, c == 0 ? q.cutA : q.cut, w->scal, w->flags, s));
-        for (unsigned it = 0; it < q.n_hio; ++it) {
-            IMR_CHECK(launch_row(it == 0 ? ROW_FIRST_CUT : ROW_MID, p, ra, s));
-            IMR_CHECK(launch_col(COL_PROJ, p, ca, s));
-        }
-        IMR_CHECK(launch_row(ROW_LAST, p, ra, s));
-        IMR_CHECK(launch_err_finalize(w->perr, w->pcnt, nrowwg, q.target, w->scal + 1, w->flags,
-                                      (unsigned *) (w->flags + 1), s));
+int ws_tune(Workspace *w, hipStream_t s) { return autotune(w->device, w->W, w->H, s, w); }
+
+int recon_begin(Workspace *w, const float *M, const Params &q, hipStream_t s)
+{
+    IMR_CHECK(ensure_weights(w, q, s));
+    IMR_HIP_CHECK(hipMemsetAsync(w->flags, 0, 64, s));
+    IMR_HIP_CHECK(hipMemsetAsync(w->scal, 0, 64, s));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
+    IMR_CHECK(launch_col(COL_INIT, p, col_args(w, M), s));
+    return launch_row(ROW_INIT, p, row_args(w, q), s);
+}
+
+int recon_cycle(Workspace *w, const float *M, const Params &q, unsigned c, hipStream_t s)
+{
+    Plan2D p;
+    IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
+    const ColArgs ca = col_args(w, M);
+    const RowArgs ra = row_args(w, q);
+    BlurArgs ba{};
+    ba.in = w->Bn;
+    ba.out = w->B2;
+    ba.wh = ba.wv = w->weights + (size_t) c * 64;
+    ba.rh = ba.rv = (w->taps[c] - 1) / 2;
+    ba.W = w->W;
+    ba.H = w->H;
+    ba.pmax = w->pmax;
+    ba.done = w->flags;
+    IMR_CHECK(launch_blur(ba, s));
+    IMR_CHECK(launch_max_finalize(w->pmax, blur_grid(w->W, w->H), c == 0 ? q.cutA : q.cut, w->scal,
+                                  w->flags, s));
+    for (unsigned it = 0; it < q.n_hio; ++it) {
+        IMR_CHECK(launch_row(it == 0 ? ROW_FIRST_CUT : ROW_MID, p, ra, s));
+        IMR_CHECK(launch_col(COL_PROJ, p, ca, s));
     }
-    IMR_CHECK(launch_copy_from_real(io_dev, w->X, n, s));
+    IMR_CHECK(launch_row(ROW_LAST, p, ra, s));
+    return launch_err_finalize(w->perr, w->pcnt, row_grid(p), q.target, w->scal + 1, w->flags,
+                               (unsigned *) (w->flags + 1), s);
+}
+
+int recon_end(Workspace *w, float *io_dev, hipStream_t s)
+{
+    return launch_copy_from_real(io_dev, w->X, w->n, s);
+}
+
+int recon_stage_results(Workspace *w, hipStream_t s)
+{
+    IMR_HIP_CHECK(hipMemcpyAsync(w->host_stage, w->scal, 16, hipMemcpyDeviceToHost, s));
+    IMR_HIP_CHECK(hipMemcpyAsync(w->host_stage + 8, w->flags, 8, hipMemcpyDeviceToHost, s));
     return IMRESH_OK;
+}
+
+void recon_read_staged(const Workspace *w, imresh_result *res, int *done)
+{
+    const float *st = w->host_stage;
+    int flags[2];
+    std::memcpy(flags, st + 8, 8);
+    if (res != nullptr) {
+        res->cycles_run = (uint32_t) flags[1];
+        res->last_error = st[1];
+        res->last_total = st[2];
+        res->last_masked = st[3];
+    }
+    if (done != nullptr) *done = flags[0];
+}
+
+}  // namespace imr
+
+namespace {
+
+int run_reconstruction(Workspace *w, float *io_dev, const Params &q, hipStream_t s)
+{
+    IMR_CHECK(recon_begin(w, io_dev, q, s));
+    for (unsigned c = 0; c < q.n_cycles; ++c) IMR_CHECK(recon_cycle(w, io_dev, q, c, s));
+    return recon_end(w, io_dev, s);
 }
 
 int read_result(Workspace *w, hipStream_t s, imresh_result *res)
 {
-    float *st = w->host_stage;
-    IMR_HIP_CHECK(hipMemcpyAsync(st, w->scal, 16, hipMemcpyDeviceToHost, s));
-    IMR_HIP_CHECK(hipMemcpyAsync(st + 8, w->flags, 8, hipMemcpyDeviceToHost, s));
+    IMR_CHECK(recon_stage_results(w, s));
     IMR_HIP_CHECK(hipStreamSynchronize(s));
-    int flags[2];
-    std::memcpy(flags, st + 8, 8);
-    res->cycles_run = (uint32_t) flags[1];
-    res->last_error = st[1];
-    res->last_total = st[2];
-    res->last_masked = st[3];
+    recon_read_staged(w, res, nullptr);
     return IMRESH_OK;
 }
 
@@ -541,6 +625,21 @@ int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream)
 
 int imresh_stream_sync(void *stream) { return from_hip(hipStreamSynchronize(S(stream))); }
 
+int imresh_stream_create(void **stream)
+{
+    if (stream == nullptr) return IMRESH_ERR_NULL;
+    hipStream_t s = nullptr;
+    IMR_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void *) s;
+    return IMRESH_OK;
+}
+
+int imresh_stream_destroy(void *stream)
+{
+    if (stream == nullptr) return IMRESH_ERR_NULL;
+    return from_hip(hipStreamDestroy(S(stream)));
+}
+
 int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
                              const imresh_params *params, void *stream, imresh_result *result)
 {
@@ -555,7 +654,8 @@ int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
     hipStream_t s = S(stream);
     Workspace *w;
     IMR_CHECK(ws_checkout(dev, width, height, s, &w));
-    int st = run_reconstruction(w, io_dev, q, s);
+    int st = ws_tune(w, s);
+    if (st == IMRESH_OK) st = run_reconstruction(w, io_dev, q, s);
     if (st == IMRESH_OK && result != nullptr) st = read_result(w, s, result);
     ws_return(w, s);
     return st;
@@ -576,7 +676,9 @@ int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
     Workspace *w;
     IMR_CHECK(ws_checkout(dev, width, height, s, &w));
     const size_t bytes = sizeof(float) * w->n;
-    int st = from_hip(hipMemcpyAsync(w->M, io_host, bytes, hipMemcpyHostToDevice, s));
+    // tuning runs on the workspace's planes, so before the modulus lands in M
+    int st = ws_tune(w, s);
+    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(w->M, io_host, bytes, hipMemcpyHostToDevice, s));
     if (st == IMRESH_OK) st = run_reconstruction(w, w->M, q, s);
     if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(io_host, w->M, bytes, hipMemcpyDeviceToHost, s));
     if (st == IMRESH_OK) {

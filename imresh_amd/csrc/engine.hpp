@@ -2,6 +2,8 @@
 // batched driver (batch.cpp) and the request server (serve.cpp).
 #pragma once
 
+#include <vector>
+
 #include "internal.hpp"
 
 namespace imr {
@@ -15,6 +17,53 @@ struct Params {
 };
 
 int normalize(const imresh_params *p, Params &q);
+
+// device state of one W x H reconstruction (one allocation: iterate X,
+// gPrev G, |g'| Bn, blurred B2, modulus M, mask, partial sums, scalars)
+struct Workspace {
+    int device = 0;
+    int W = 0, H = 0;
+    size_t n = 0;
+    size_t bytes = 0;           // device bytes of the main allocation
+    void *base = nullptr;
+    float2 *X = nullptr, *G = nullptr;
+    float *Bn = nullptr, *B2 = nullptr, *M = nullptr;
+    uint8_t *mask = nullptr;
+    float *pmax = nullptr;
+    float *perr = nullptr;
+    unsigned *pcnt = nullptr;
+    float *scal = nullptr;      // [0] threshold, [1..3] error, total, nmasked
+    int *flags = nullptr;       // [0] done, [1] cycles
+    float *weights = nullptr;   // [cycles][64]
+    unsigned wcap = 0;
+    std::vector<int> taps;
+    float w_sigma0 = -1.f, w_dsigma = -1.f;
+    unsigned w_ncycles = 0;
+    std::vector<float> host_weights;   // host copy of the weights table
+    float *host_stage = nullptr;  // pinned, 64 floats + readback
+    hipEvent_t ready = nullptr;   // recorded when the last user's work is queued
+    bool pending = false;
+};
+
+int ws_alloc(int device, int W, int H, Workspace **out);
+void ws_release(Workspace *w);
+// device bytes of a workspace (for HBM budgets)
+size_t ws_bytes(int W, int H);
+int ensure_weights(Workspace *w, const Params &q, hipStream_t s);
+
+// one reconstruction in steps (all asynchronous on s): begin = weights,
+// state reset, initial inverse transform of the modulus M (device);
+// cycle c = blur, threshold, n_hio fused iterations, error / early exit;
+// end = Re(g') -> io_dev.  stage_results copies the scalars to the pinned
+// host stage, read_staged decodes them once that copy has completed.
+// picks the size's kernel variants on first use; it overwrites the
+// workspace's planes (M included), so it runs before the modulus is staged
+int ws_tune(Workspace *w, hipStream_t s);
+int recon_begin(Workspace *w, const float *M, const Params &q, hipStream_t s);
+int recon_cycle(Workspace *w, const float *M, const Params &q, unsigned c, hipStream_t s);
+int recon_end(Workspace *w, float *io_dev, hipStream_t s);
+int recon_stage_results(Workspace *w, hipStream_t s);
+void recon_read_staged(const Workspace *w, imresh_result *res, int *done);
 int current_device(int *dev);
 // picks (once per device and size) the row / column kernel variants
 int tune_size(int device, int W, int H, hipStream_t s);

@@ -49,6 +49,8 @@ struct RowArgs {
     const float2 *tw;      // row-length twiddle table (set by launch_row)
     int batch;             // images (grid.y; 0 or 1: one image)
     size_t istride;        // elements between consecutive images
+    int len;               // row length W (Bluestein kernels)
+    const float2 *bs;      // Bluestein tables of W (set by launch_row)
 };
 
 struct ColArgs {
@@ -62,6 +64,8 @@ struct ColArgs {
     const float2 *tw;      // column-length twiddle table (set by launch_col)
     int batch;             // images (grid.y; 0 or 1: one image)
     size_t istride;        // elements between consecutive images
+    int len;               // column length H (Bluestein kernels)
+    const float2 *bs;      // Bluestein tables of H (set by launch_col)
 };
 
 struct BlurArgs {
@@ -88,6 +92,9 @@ constexpr int kFlagStride = 2;
 constexpr int kBlurWeightRow = 64;
 
 // A compile-time plan (plans.hpp) of one line length with its twiddles.
+// A line length is transformed either by its own compiled plan or, when it
+// has none, by the Bluestein kernels of the power-of-two convolution length
+// m >= 2n-1 (group, T, L, C and tw then describe the length-m plan).
 struct LinePlan {
     int n;
     int group;       // translation-unit group of the kernel instantiations
@@ -95,7 +102,14 @@ struct LinePlan {
     int L;           // rows per row workgroup
     int C;           // columns per column workgroup
     const float2 *tw;
+    int blue;        // 1: Bluestein
+    int m;           // convolution length (Bluestein)
+    const float2 *bs;   // Bluestein tables (device)
 };
+
+// variant index of the Bluestein kernels (one shape per convolution length)
+constexpr int kBlueVariant = 1000;
+constexpr int kMaxLine = 4096;   // longest line a Bluestein plan serves
 
 struct Plan2D {
     int W, H;

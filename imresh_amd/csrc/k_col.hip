@@ -11,6 +11,7 @@
 //            (cudaShrinkWrap.cu:229-232).
 //   CK_FWD   forward column transform of the 2-D FFT entry point.
 #include "internal.hpp"
+#include "blue_device.hpp"
 #include "ops_device.hpp"
 #include "plans.hpp"
 
@@ -30,7 +31,7 @@ enum ColKernel { CK_FWD, CK_INV, CK_PROJ };
 // LDS one after the other (half the LDS of a complex tile); PF (CK_PROJ
 // only): the measured modulus is read right after the column gather instead
 // of at the junction.
-template <class P, int C, bool SPLIT, int K, bool PF = false>
+template <class P, int C, bool SPLIT, int K, bool PF = false, bool BL = false>
 __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 {
     const int b = blockIdx.y;   // image of a batch
@@ -54,13 +55,14 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
     const Lds L{(decltype(Lds::p)) smem, c};
     const cf *tw = (const cf *) a.tw;
     cf v[P::E];
+    const Blue bl{BL ? a.len : P::n, (const cf *) a.bs};
 
     if constexpr (K == CK_FWD) {
         const cf *in = (const cf *) a.in + wb;
         cf *out = (cf *) a.out + wb;
         auto g0 = [=](int e) { return at32(in, e * W + co); };
         auto sl = [=](int e, cf x) { if (valid) at32(out, e * W + co) = x; };
-        line_fft<P, -1>(v, tw, t, L, g0, sl);
+        xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
     } else if constexpr (K == CK_INV) {
         const bool real_in = (a.flags & kColRealIn) != 0;
         const cf *in = (const cf *) a.in + wb;
@@ -68,7 +70,7 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
         cf *out = (cf *) a.out + wb;
         auto g0 = [=](int e) { return real_in ? mkc(at32(M, e * W + co), 0.f) : at32(in, e * W + co); };
         auto sl = [=](int e, cf x) { if (valid) at32(out, e * W + co) = x; };
-        line_fft<P, +1>(v, tw, t, L, g0, sl);
+        xfft<P, +1, BL>(v, tw, t, L, bl, g0, sl);
     } else if constexpr (K == CK_PROJ) {
         cf *X = (cf *) a.X + wb;
         const float *M = a.M + wb;
@@ -82,7 +84,7 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
             else return modulus_project_fused(z, at32(M, e * W + co));
         };
         auto sl = [=](int e, cf x) { if (valid) at32(X, e * W + co) = x; };
-        line_pair<P, -1, +1>(v, tw, t, L, g0, pre, mid, sl);
+        xpair<P, -1, +1, BL>(v, tw, t, L, bl, g0, pre, mid, sl);
     }
 }
 
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 template <class P, int C, bool SPLIT>
 constexpr size_t col_lds() { return (SPLIT ? sizeof(float) : sizeof(cf)) * (size_t) lds_elems(C * P::n); }
 
-template <class P, int C, bool SPLIT, bool PF = false>
+template <class P, int C, bool SPLIT, bool PF = false, bool BL = false>
 int launch_variant(int mode, ColArgs a, hipStream_t s)
 {
     static_assert(C * P::T <= 1024, "column workgroup too large");
@@ -100,9 +102,9 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
     if constexpr (lds > 64 * 1024) {
         static const hipError_t attr = [&] {
             hipError_t e = hipSuccess;
-            for (const void *f : {(const void *) k_col<P, C, SPLIT, CK_FWD>,
-                                  (const void *) k_col<P, C, SPLIT, CK_INV>,
-                                  (const void *) k_col<P, C, SPLIT, CK_PROJ, PF>})
+            for (const void *f : {(const void *) k_col<P, C, SPLIT, CK_FWD, false, BL>,
+                                  (const void *) k_col<P, C, SPLIT, CK_INV, false, BL>,
+                                  (const void *) k_col<P, C, SPLIT, CK_PROJ, PF, BL>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -112,21 +114,21 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
     switch (mode) {
     case COL_FWD:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_FWD>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INV:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INIT:
         a.flags = kColRealIn;
         a.in = a.X;   // unused with kColRealIn; never a null pointer in a kernel
         a.out = a.X;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
         break;
     case COL_PROJ:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ, PF>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ, PF, BL>), grid, block, lds, s, a);
         break;
     default: return IMRESH_ERR_INVALID;
     }
@@ -151,10 +153,15 @@ constexpr int col_shape_count()
 template <int N, int... Rs>
 constexpr int col_variant_count() { return 2 * col_shape_count<N, Rs...>(); }
 
+template <int N>
+constexpr bool blue_length() { return N >= 64 && N <= 2 * kMaxLine && (N & (N - 1)) == 0; }
+
 template <int N, int... Rs>
 int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
 {
     using P16 = PlanE<16, N, Rs...>;
+    if constexpr (blue_length<N>())
+        if (var == kBlueVariant) return launch_variant<P16, P16::C, false, false, true>(mode, a, s);
     constexpr int ns = col_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;

@@ -15,6 +15,7 @@
 //             blur (:240) and the HIO error sums (cudaVectorReduce.cu:284-355).
 //   RK_FWD / RK_INV  plain row transforms of the 2-D FFT entry point.
 #include "internal.hpp"
+#include "blue_device.hpp"
 #include "ops_device.hpp"
 #include "plans.hpp"
 
@@ -29,8 +30,9 @@ namespace {
 enum RowKernel { RK_FWD, RK_INV, RK_INVOUT, RK_FIRST, RK_MID };
 
 // NL rows per workgroup; PF (RK_MID only): gPrev and the mask are read right
-// after the row gather instead of at the junction
-template <class P, int NL, int K, bool PF = false>
+// after the row gather instead of at the junction; BL: Bluestein kernels of
+// row length a.len with P the convolution plan
+template <class P, int NL, int K, bool PF = false, bool BL = false>
 __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
 {
     // image b of a batch (grid.y): its planes start b * istride elements in,
@@ -38,7 +40,9 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     const int b = blockIdx.y;
     if (a.done != nullptr && a.done[b * kFlagStride]) return;
     extern __shared__ cf smem[];
-    constexpr int n = P::n, T = P::T;
+    constexpr int T = P::T;
+    const int n = BL ? a.len : P::n;
+    const Blue bl{n, (const cf *) a.bs};
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
     const int row = blockIdx.x * NL + l;
     const bool valid = row < a.H;
@@ -46,7 +50,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     // read row 0 of the workgroup and never store
     const size_t wb = (size_t) b * a.istride + (size_t) blockIdx.x * NL * n;
     const int lo = valid ? l * n : 0;
-    const LdsC<1> L{smem, l * n};
+    const LdsC<1> L{smem, l * P::n};
     const cf *tw = (const cf *) a.tw;
     cf v[P::E];
     cf *X = (cf *) a.X + wb;
@@ -58,8 +62,8 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         cf *out = (cf *) a.out + wb;
         auto g0 = [=](int e) { return at32(in, lo + e); };
         auto sl = [=](int e, cf x) { if (valid) at32(out, lo + e) = x; };
-        if constexpr (K == RK_FWD) line_fft<P, -1>(v, tw, t, L, g0, sl);
-        else line_fft<P, +1>(v, tw, t, L, g0, sl);
+        if constexpr (K == RK_FWD) xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
+        else xfft<P, +1, BL>(v, tw, t, L, bl, g0, sl);
     } else if constexpr (K == RK_INVOUT) {
         __shared__ float red_f[NL * P::T];
         __shared__ unsigned red_u[NL * P::T];
@@ -82,7 +86,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
                 }
             }
         };
-        line_fft<P, +1>(v, tw, t, L, ldX, sl);
+        xfft<P, +1, BL>(v, tw, t, L, bl, ldX, sl);
         if (fl & kRowError) {
             err = block_reduce(err, red_f, [](float x, float y) { return x + y; });
             cnt = block_reduce(cnt, red_u, [](unsigned x, unsigned y) { return x + y; });
@@ -111,7 +115,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             if (valid) at32(G, lo + e) = gp;
             return gp;
         };
-        line_fft<P, -1>(v, tw, t, L, g0, stX);
+        xfft<P, -1, BL>(v, tw, t, L, bl, g0, stX);
     } else if constexpr (K == RK_MID) {
         cf *G = (cf *) a.G + wb;
         const uint8_t *mask = a.mask + wb;
@@ -131,11 +135,11 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             if (valid) at32(G, lo + e) = gp;
             return gp;
         };
-        line_pair<P, +1, -1>(v, tw, t, L, ldX, pre, mid, stX);
+        xpair<P, +1, -1, BL>(v, tw, t, L, bl, ldX, pre, mid, stX);
     }
 }
 
-template <class P, int NL, bool PF = false>
+template <class P, int NL, bool PF = false, bool BL = false>
 int launch_variant(int mode, RowArgs a, hipStream_t s)
 {
     static_assert(NL * P::T <= 1024, "row workgroup too large");
@@ -145,10 +149,11 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
     if constexpr (lds > 64 * 1024) {
         static const hipError_t attr = [&] {
             hipError_t e = hipSuccess;
-            for (const void *f : {(const void *) k_row<P, NL, RK_FWD>, (const void *) k_row<P, NL, RK_INV>,
-                                  (const void *) k_row<P, NL, RK_INVOUT>,
-                                  (const void *) k_row<P, NL, RK_FIRST>,
-                                  (const void *) k_row<P, NL, RK_MID, PF>})
+            for (const void *f : {(const void *) k_row<P, NL, RK_FWD, false, BL>,
+                                  (const void *) k_row<P, NL, RK_INV, false, BL>,
+                                  (const void *) k_row<P, NL, RK_INVOUT, false, BL>,
+                                  (const void *) k_row<P, NL, RK_FIRST, false, BL>,
+                                  (const void *) k_row<P, NL, RK_MID, PF, BL>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -156,29 +161,29 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         if (attr != hipSuccess) return from_hip(attr);
     }
     switch (mode) {
-    case ROW_FWD: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_FWD>), grid, block, lds, s, a); break;
-    case ROW_INV: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_INV>), grid, block, lds, s, a); break;
+    case ROW_FWD: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
+    case ROW_INV: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_INV, false, BL>), grid, block, lds, s, a); break;
     case ROW_INIT:
         a.flags = kRowWriteG | kRowWriteBn;
-        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_LAST:
         a.flags = kRowWriteBn | kRowError;
-        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_LAST_PLAIN:
         a.flags = 0;
-        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_FIRST_CUT:
         a.flags = kRowCut;
-        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_FIRST_MASK:
         a.flags = 0;
-        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
         break;
-    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID, PF>), grid, block, lds, s, a); break;
+    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID, PF, BL>), grid, block, lds, s, a); break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());
@@ -202,10 +207,16 @@ constexpr int row_shape_count()
 template <int N, int... Rs>
 constexpr int row_variant_count() { return 2 * row_shape_count<N, Rs...>(); }
 
+// Bluestein kernels exist for the power-of-two convolution lengths
+template <int N>
+constexpr bool blue_length() { return N >= 64 && N <= 2 * kMaxLine && (N & (N - 1)) == 0; }
+
 template <int N, int... Rs>
 int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
 {
     using P16 = PlanE<16, N, Rs...>;
+    if constexpr (blue_length<N>())
+        if (var == kBlueVariant) return launch_variant<P16, P16::L, false, true>(mode, a, s);
     constexpr int ns = row_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;
@@ -226,6 +237,7 @@ template <int N, int... Rs>
 int row_variant_rows(int var)
 {
     using P16 = PlanE<16, N, Rs...>;
+    if (var == kBlueVariant) return P16::L;
     constexpr int ns = row_shape_count<N, Rs...>();
     const int shape = var >= ns ? var - ns : var;
     if constexpr (N >= 1024) {

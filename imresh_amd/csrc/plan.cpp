@@ -4,6 +4,7 @@
 // of the reference (computed on the host exactly as the reference does, so
 // the device blur uses bit-identical weights).
 #include <cmath>
+#include <complex>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -17,7 +18,7 @@ namespace imr {
 
 namespace {
 
-std::mutex g_plan_mtx;
+std::recursive_mutex g_plan_mtx;   // get_line_plan recurses for Bluestein lengths
 std::map<std::pair<int, int>, LinePlan> g_lines;   // (device, n) -> plan (twiddles on device)
 
 template <int G, int N, int... Rs>
@@ -65,20 +66,103 @@ void append_twiddles(const std::vector<int> &rad, std::vector<float2> &tw)
     }
 }
 
+// forward DFT of a power-of-two length in double (iterative radix 2), for
+// the Bluestein tables
+void dft_pow2(std::vector<std::complex<double>> &a)
+{
+    const size_t m = a.size();
+    for (size_t i = 1, j = 0; i < m; ++i) {
+        size_t bit = m >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= m; len <<= 1) {
+        for (size_t i = 0; i < m; i += len)
+            for (size_t k = 0; k < len / 2; ++k) {
+                const double ang = -2.0 * M_PI * (double) k / (double) len;
+                const std::complex<double> w(std::cos(ang), std::sin(ang));
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+    }
+}
+
+// convolution length of a Bluestein line: smallest power of two >= 2n-1
+// with compiled kernels (at least 64)
+int blue_m(int n)
+{
+    int m = 64;
+    while (m < 2 * n - 1) m <<= 1;
+    return m;
+}
+
+// Bluestein tables of length n (layout in blue_device.hpp), float from double
+std::vector<float2> blue_tables(int n, int m)
+{
+    std::vector<std::complex<double>> a(n), bf(m, 0.0), bi(m, 0.0);
+    for (int j = 0; j < n; ++j) {
+        // j^2 mod 2n keeps the angle exact for large j
+        const long long q = ((long long) j * j) % (2LL * n);
+        const double ang = -M_PI * (double) q / (double) n;
+        a[j] = std::complex<double>(std::cos(ang), std::sin(ang));
+    }
+    for (int j = 0; j < n; ++j) {
+        bf[j] = std::conj(a[j]);
+        bi[j] = a[j];
+        if (j > 0) {
+            bf[m - j] = std::conj(a[j]);
+            bi[m - j] = a[j];
+        }
+    }
+    dft_pow2(bf);
+    dft_pow2(bi);
+    std::vector<float2> out;
+    out.reserve((size_t) n + 2 * m);
+    for (int j = 0; j < n; ++j) out.push_back(make_float2((float) a[j].real(), (float) a[j].imag()));
+    for (int k = 0; k < m; ++k) out.push_back(make_float2((float) (bf[k].real() / m), (float) (bf[k].imag() / m)));
+    for (int k = 0; k < m; ++k) out.push_back(make_float2((float) (bi[k].real() / m), (float) (bi[k].imag() / m)));
+    return out;
+}
+
 }  // namespace
 
-bool line_supported(int n) { return find_info(n) != nullptr; }
+bool line_supported(int n) { return find_info(n) != nullptr || (n >= 1 && n <= kMaxLine); }
 
 int get_line_plan(int device, int n, LinePlan *out)
 {
-    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    std::lock_guard<std::recursive_mutex> lk(g_plan_mtx);
     auto it = g_lines.find({device, n});
     if (it != g_lines.end()) {
         *out = it->second;
         return IMRESH_OK;
     }
     const PlanInfo *pi = find_info(n);
-    if (pi == nullptr) return IMRESH_ERR_UNSUPPORTED_SIZE;
+    if (pi == nullptr) {
+        if (n < 1 || n > kMaxLine) return IMRESH_ERR_UNSUPPORTED_SIZE;
+        // Bluestein: the convolution plan's kernels and twiddles + the tables
+        const int m = blue_m(n);
+        LinePlan lm;
+        IMR_CHECK(get_line_plan(device, m, &lm));
+        const std::vector<float2> tab = blue_tables(n, m);
+        float2 *dev = nullptr;
+        int prev = 0;
+        hipGetDevice(&prev);
+        hipSetDevice(device);
+        hipError_t e = hipMalloc(&dev, sizeof(float2) * tab.size());
+        if (e == hipSuccess) e = hipMemcpy(dev, tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice);
+        hipSetDevice(prev);
+        if (e != hipSuccess) return from_hip(e);
+        LinePlan lp = lm;
+        lp.n = n;
+        lp.blue = 1;
+        lp.m = m;
+        lp.bs = dev;
+        g_lines[{device, n}] = lp;
+        *out = lp;
+        return IMRESH_OK;
+    }
     std::vector<int> fwd(pi->rad, pi->rad + pi->np), inv(fwd.rbegin(), fwd.rend());
     std::vector<float2> tw;
     append_twiddles(fwd, tw);
@@ -110,13 +194,13 @@ std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tuned;   // (device, 
 
 void set_tuned_variants(int device, int W, int H, int rowVar, int colVar)
 {
-    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    std::lock_guard<std::recursive_mutex> lk(g_plan_mtx);
     g_tuned[{device, W, H}] = {rowVar, colVar};
 }
 
 bool get_tuned_variants(int device, int W, int H, int *rowVar, int *colVar)
 {
-    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    std::lock_guard<std::recursive_mutex> lk(g_plan_mtx);
     auto it = g_tuned.find({device, W, H});
     if (it == g_tuned.end()) return false;
     *rowVar = it->second.first;
@@ -140,12 +224,14 @@ int get_plan2d(int device, int W, int H, Plan2D *out)
 int row_grid(const Plan2D &p)
 {
     int L = 1;
+    const int n = p.row.blue ? p.row.m : p.W;
+    const int v = p.row.blue ? kBlueVariant : p.rowVar;
     switch (p.row.group) {
-    case 0: L = row_rows_g0(p.W, p.rowVar); break;
-    case 1: L = row_rows_g1(p.W, p.rowVar); break;
-    case 2: L = row_rows_g2(p.W, p.rowVar); break;
-    case 3: L = row_rows_g3(p.W, p.rowVar); break;
-    case 4: L = row_rows_g4(p.W, p.rowVar); break;
+    case 0: L = row_rows_g0(n, v); break;
+    case 1: L = row_rows_g1(n, v); break;
+    case 2: L = row_rows_g2(n, v); break;
+    case 3: L = row_rows_g3(n, v); break;
+    case 4: L = row_rows_g4(n, v); break;
     default: break;
     }
     return (p.H + L - 1) / L;
@@ -154,13 +240,16 @@ int row_grid(const Plan2D &p)
 int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
 {
     a.H = p.H;
-    const int v = p.rowVar;
+    a.len = p.W;
+    a.bs = p.row.bs;
+    const int n = p.row.blue ? p.row.m : p.W;
+    const int v = p.row.blue ? kBlueVariant : p.rowVar;
     switch (p.row.group) {
-    case 0: return launch_row_g0(mode, v, p.W, p.row, a, s);
-    case 1: return launch_row_g1(mode, v, p.W, p.row, a, s);
-    case 2: return launch_row_g2(mode, v, p.W, p.row, a, s);
-    case 3: return launch_row_g3(mode, v, p.W, p.row, a, s);
-    case 4: return launch_row_g4(mode, v, p.W, p.row, a, s);
+    case 0: return launch_row_g0(mode, v, n, p.row, a, s);
+    case 1: return launch_row_g1(mode, v, n, p.row, a, s);
+    case 2: return launch_row_g2(mode, v, n, p.row, a, s);
+    case 3: return launch_row_g3(mode, v, n, p.row, a, s);
+    case 4: return launch_row_g4(mode, v, n, p.row, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
     }
 }
@@ -168,19 +257,23 @@ int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
 int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
 {
     a.W = p.W;
-    const int v = p.colVar;
+    a.len = p.H;
+    a.bs = p.col.bs;
+    const int n = p.col.blue ? p.col.m : p.H;
+    const int v = p.col.blue ? kBlueVariant : p.colVar;
     switch (p.col.group) {
-    case 0: return launch_col_g0(mode, v, p.H, p.col, a, s);
-    case 1: return launch_col_g1(mode, v, p.H, p.col, a, s);
-    case 2: return launch_col_g2(mode, v, p.H, p.col, a, s);
-    case 3: return launch_col_g3(mode, v, p.H, p.col, a, s);
-    case 4: return launch_col_g4(mode, v, p.H, p.col, a, s);
+    case 0: return launch_col_g0(mode, v, n, p.col, a, s);
+    case 1: return launch_col_g1(mode, v, n, p.col, a, s);
+    case 2: return launch_col_g2(mode, v, n, p.col, a, s);
+    case 3: return launch_col_g3(mode, v, n, p.col, a, s);
+    case 4: return launch_col_g4(mode, v, n, p.col, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
     }
 }
 
 int row_variants(const LinePlan &lp)
 {
+    if (lp.blue) return 1;
     switch (lp.group) {
     case 0: return row_variants_g0(lp.n);
     case 1: return row_variants_g1(lp.n);
@@ -193,6 +286,7 @@ int row_variants(const LinePlan &lp)
 
 int col_variants(const LinePlan &lp)
 {
+    if (lp.blue) return 1;
     switch (lp.group) {
     case 0: return col_variants_g0(lp.n);
     case 1: return col_variants_g1(lp.n);

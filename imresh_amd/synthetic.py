@@ -41,18 +41,27 @@ def make_object(width: int, height: int, n_shapes: int, central: int,
     obj = np.zeros((height, width), dtype=np.float64)
     c = int(min(central, width, height))
     x0, y0 = (width - c) // 2, (height - c) // 2
-    yy, xx = np.mgrid[0:c, 0:c]
     region = obj[y0:y0 + c, x0:x0 + c]
+
+    def box(center, half):
+        # pixel range that can satisfy the shape test (one pixel of margin)
+        return max(0, int(np.floor(center - half)) - 1), min(c, int(np.ceil(center + half)) + 2)
+
     for s in range(n_shapes):
         cx, cy = rng.uniform(0, c, size=2)
         amp = rng.uniform(0.2, 1.0)
         if s % 2 == 0:
             hx, hy = rng.uniform(0.02, 0.12, size=2) * c
+            (ax, bx), (ay, by) = box(cx, hx), box(cy, hy)
+            yy, xx = np.mgrid[ay:by, ax:bx]
             m = (np.abs(xx - cx) <= hx) & (np.abs(yy - cy) <= hy)
         else:
             r = rng.uniform(0.02, 0.12) * c
+            (ax, bx), (ay, by) = box(cx, r), box(cy, r)
+            yy, xx = np.mgrid[ay:by, ax:bx]
             m = (xx - cx) ** 2 + (yy - cy) ** 2 <= r * r
-        np.maximum(region, np.where(m, amp, 0.0), out=region)
+        sub = region[ay:by, ax:bx]
+        np.maximum(sub, np.where(m, amp, 0.0), out=sub)
     return obj
 
 
@@ -93,6 +102,63 @@ def config_2048(index: int = 0):
 def config_8192(index: int = 0):
     """8192 x 8192, 512 shapes in the central 4096 x 4096, 1e11 photons."""
     return workload(8192, 8192, n_shapes=512, central=4096, photons=1e11, index=index)
+
+
+BATCH_SIZES = (64, 96, 128, 200, 256, 384, 512, 768, 1024)
+
+
+def batch_shapes(n: int) -> tuple[int, int]:
+    """Central region n/2 and 8 shapes per 128^2 of it (at least 2)."""
+    central = n // 2
+    return max(2, int(round(8 * (central / 128.0) ** 2))), central
+
+
+def batch_sizes(count: int = 512, index: int = 0) -> list[int]:
+    """Image sizes of the batched workload (uniform over BATCH_SIZES)."""
+    rng = rng_for(10_000_000 + index)
+    return [int(BATCH_SIZES[k]) for k in rng.integers(0, len(BATCH_SIZES), count)]
+
+
+def config_batch(count: int = 512, index: int = 0):
+    """The batched workload: `count` square images, sizes uniform over
+    BATCH_SIZES (a log-spaced set, so uniform over it is log-uniform), each
+    with batch_shapes(n) shapes and 1e7 photons.  Returns (sizes, objects,
+    moduli)."""
+    sizes = batch_sizes(count, index)
+    objs, mods = [], []
+    for i, n in enumerate(sizes):
+        shapes, central = batch_shapes(n)
+        o, m = workload(n, n, n_shapes=shapes, central=central, photons=1e7,
+                        index=20_000_000 + 1000 * index + i)
+        objs.append(o)
+        mods.append(m)
+    return sizes, objs, mods
+
+
+def fast_modulus(n: int, index: int, photons: float = 1e9) -> np.ndarray:
+    """A serving request's input: the object of batch_shapes(n), intensity
+    from a complex64 FFT (scipy, all host threads: input generation only),
+    Poisson-sampled at `photons`, modulus = sqrt."""
+    import scipy.fft
+    rng = rng_for(30_000_000 + index)
+    shapes, central = batch_shapes(n)
+    obj = make_object(n, n, shapes, central, rng)
+    F = scipy.fft.fft2(obj.astype(np.complex64), workers=-1)
+    I = (F.real.astype(np.float64) ** 2 + F.imag.astype(np.float64) ** 2)
+    I = poisson_sample(I, photons, rng)
+    return np.sqrt(I).astype(np.float32)
+
+
+def serve_trace(count: int, rate: float = 200.0, lo: int = 128, hi: int = 2048, index: int = 0):
+    """The serving workload: Poisson arrivals at `rate` per second, square
+    sizes round(exp(U[ln lo, ln hi])), 5 / 10 / 20 cycles of 20 HIO
+    iterations with equal probability.  Returns [(arrival_s, n, cycles)]."""
+    rng = rng_for(40_000_000 + index)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    t -= t[0]
+    sizes = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+    cycles = rng.choice([5, 10, 20], count)
+    return [(float(a), int(n), int(c)) for a, n, c in zip(t, sizes, cycles)]
 
 
 # ------------------------------------------------------------------ metrics

@@ -87,6 +87,11 @@ int imresh_memcpy_d2h(void *host, const void *dev, uint64_t bytes, void *stream)
 /* asynchronous device-to-device copy on `stream` */
 int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream);
 int imresh_stream_sync(void *stream);
+/* A non-blocking HIP stream at the default priority (for callers without a
+ * HIP runtime of their own, e.g. to overlap reconstructions), and its
+ * destruction. */
+int imresh_stream_create(void **stream);
+int imresh_stream_destroy(void *stream);
 
 /* ---- whole reconstruction ---------------------------------------------- */
 /* Replaces imresh::algorithms::shrinkWrap (src/imresh/algorithms/shrinkWrap.cpp:111-361,
@@ -126,6 +131,52 @@ int imresh_shrinkwrap_batch(const imresh_image *images, uint32_t count,
 int imresh_shrinkwrap_batch_device(const imresh_image *images, uint32_t count,
                                    const imresh_params *params, void *stream,
                                    imresh_result *results);
+
+/* ---- reconstruction server (no reference counterpart: the reference's
+ * task queue, taskQueue.cu:91-193, runs each request on a fresh host thread
+ * and stream slot with no device state kept between calls) -------------- */
+typedef struct imresh_server imresh_server;
+
+typedef struct imresh_server_completion {
+    uint64_t      id;           /* from imresh_server_submit               */
+    int32_t       status;       /* IMRESH_OK or an error code              */
+    imresh_result result;
+    double        latency_ms;   /* submit -> completion observed           */
+} imresh_server_completion;
+
+typedef struct imresh_server_info {
+    uint64_t submitted, completed, queued, active;
+    uint64_t bytes_in_use;      /* device bytes held (active + cached)      */
+    uint64_t peak_bytes;
+    uint64_t budget;
+} imresh_server_info;
+
+/* Called on the server's scheduler thread when a request completes; io is
+ * the request's buffer (Re(g') written).  Must not block for long. */
+typedef void (*imresh_server_callback)(void *user, const imresh_server_completion *c, float *io);
+
+/* Continuous batching on the current device: requests of any supported
+ * size are admitted in arrival order while fewer than max_active run and
+ * their device state fits hbm_budget_bytes (0: 200 GiB; max_active 0: 64;
+ * lanes = HIP streams, 0: 8); every admitted request advances cycle by cycle
+ * with its state resident on the device, interleaved with the others.  Each
+ * result equals imresh_shrinkwrap on the same input. */
+int imresh_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t lanes,
+                         imresh_server **server);
+/* Queues one reconstruction of io (|F| in, Re(g') out; device memory when
+ * device_io != 0, else host).  The buffer must stay valid until completion.
+ * IMRESH_ERR_BUDGET when the request alone exceeds the budget. */
+int imresh_server_submit(imresh_server *server, float *io, uint32_t width, uint32_t height,
+                         const imresh_params *params, int device_io,
+                         imresh_server_callback callback, void *user, uint64_t *id);
+/* Moves up to capacity completion records (oldest first) into out. */
+int imresh_server_poll(imresh_server *server, imresh_server_completion *out, uint32_t capacity,
+                       uint32_t *count);
+/* Blocks until every submitted request has completed. */
+int imresh_server_drain(imresh_server *server);
+int imresh_server_stats(imresh_server *server, imresh_server_info *info);
+/* Drains, stops the scheduler and frees the device state. */
+int imresh_server_destroy(imresh_server *server);
 
 /* HIO inner loop from a given device state, n_iter iterations
  * (cudaShrinkWrap.cu:311-365 / shrinkWrap.cpp:260-319): gprime (complex, g'),

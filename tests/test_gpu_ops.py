@@ -320,6 +320,11 @@ def test_fft2d_closed_forms():
 
 
 def test_fft_unsupported_size_is_an_error():
-    d = B(8 * 7 * 7)
-    assert L().imresh_fft2d(d.ptr, d.ptr, 7, 7, -1, None) == 3   # IMRESH_ERR_UNSUPPORTED_SIZE
-    assert L().imresh_fft_supported(2048) == 1 and L().imresh_fft_supported(7) == 0
+    """Every length up to 4096 has a transform (own plan or Bluestein); the
+    compiled plans also cover 8192; anything else is an error."""
+    d = B(8 * 4097 * 2)
+    assert L().imresh_fft2d(d.ptr, d.ptr, 4097, 2, -1, None) == 3   # IMRESH_ERR_UNSUPPORTED_SIZE
+    assert L().imresh_fft2d(d.ptr, d.ptr, 2, 4097, -1, None) == 3
+    assert L().imresh_fft_supported(2048) == 1 and L().imresh_fft_supported(7) == 1
+    assert L().imresh_fft_supported(4096) == 1 and L().imresh_fft_supported(8192) == 1
+    assert L().imresh_fft_supported(4097) == 0 and L().imresh_fft_supported(0) == 0
