@@ -140,21 +140,31 @@ def batch_bench(L, sizes, mods, reps: int = 2, plain_streams: int = 3) -> dict:
         for st in streams:
             im.sync(st)
 
-    log("[bench] batch warmup (autotunes every size)")
-    restore(outs_b)
-    im.shrink_wrap_batch_device(images, p, want_results=False)
-    restore(outs_p)
-    plain()
-    tb, tp = [], []
-    for _ in range(reps):
+    # both paths timed with per-cycle graph replay and with eager launches;
+    # each reports its faster mode
+    modes = {}
+    for graphs in (1, 0):
+        im.set_option(im.OPT_CYCLE_GRAPHS, graphs)
+        log(f"[bench] batch vs plain loop, cycle graphs {'on' if graphs else 'off'} (warmup first)")
         restore(outs_b)
-        t0 = time.perf_counter()
         im.shrink_wrap_batch_device(images, p, want_results=False)
-        tb.append(time.perf_counter() - t0)
         restore(outs_p)
-        t0 = time.perf_counter()
         plain()
-        tp.append(time.perf_counter() - t0)
+        tb, tp = [], []
+        for _ in range(reps):
+            restore(outs_b)
+            t0 = time.perf_counter()
+            im.shrink_wrap_batch_device(images, p, want_results=False)
+            tb.append(time.perf_counter() - t0)
+            restore(outs_p)
+            t0 = time.perf_counter()
+            plain()
+            tp.append(time.perf_counter() - t0)
+        modes[graphs] = (min(tb), min(tp))
+    im.set_option(im.OPT_CYCLE_GRAPHS, 1)
+    gb = min(modes, key=lambda g: modes[g][0])
+    gp = min(modes, key=lambda g: modes[g][1])
+    tb, tp = [modes[gb][0]], [modes[gp][1]]
     same = all(np.array_equal(a.download(np.float32, m.shape), b.download(np.float32, m.shape))
                for a, b, m in zip(outs_b, outs_p, mods))
     for st in streams:
@@ -166,6 +176,9 @@ def batch_bench(L, sizes, mods, reps: int = 2, plain_streams: int = 3) -> dict:
             "elements": elems, "photons": 1e7, "batched_recon_per_s": round(rb, 2),
             "plain_loop_recon_per_s": round(rp, 2), "plain_loop_streams": plain_streams,
             "speedup": round(rb / rp, 3), "batched_s": round(min(tb), 4), "plain_s": round(min(tp), 4),
+            "batched_cycle_graphs": bool(gb), "plain_cycle_graphs": bool(gp),
+            "seconds_by_mode": {("graphs" if g else "eager"): {"batched": round(v[0], 4), "plain": round(v[1], 4)}
+                                for g, v in modes.items()},
             "batched_alg_GBps_80B": round(80 * elems * N_CYCLES * N_HIO / min(tb) / 1e9, 1),
             "outputs_bitwise_equal": bool(same)}
 
@@ -206,43 +219,50 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200) -> d
         wait()
         return t0, ios
 
-    # continuous-batching server
-    done_t = {}
-    with im.Server(hbm_budget_bytes=budget_gib << 30) as srv:
-        ids = {}
+    def run_server():
+        with im.Server(hbm_budget_bytes=budget_gib << 30) as srv:
+            ids = {}
 
-        def sub(k, io):
-            ids[srv.submit(io, params[k])] = k
+            def sub(k, io):
+                ids[srv.submit(io, params[k])] = k
 
-        t0, ios_s = replay(sub, srv.drain)
-        comp = srv.completions(count + 16)
-        peak = srv.stats().peak_bytes
-    # completion wall time per request from submit-relative latency
-    sub_lat = {ids[c[0]]: c[3] for c in comp}
-    bad = sum(1 for c in comp if c[1] != 0)
-    # latency measured from the scheduled arrival: the submit of request k
-    # happened at or after its arrival, latency_ms counts from the submit
-    lat_s = [sub_lat[k] for k in range(count)]
-    span_s = max(trace[k][0] + sub_lat[k] / 1e3 for k in range(count))
+            _, ios = replay(sub, srv.drain)
+            comp = srv.completions(count + 16)
+            peak = srv.stats().peak_bytes
+        lat = {ids[c[0]]: c[3] for c in comp}   # from submit (= arrival: submit never blocks)
+        errs = sum(1 for c in comp if c[1] != 0)
+        span = max(trace[k][0] + lat[k] / 1e3 for k in range(count))
+        return [lat[k] for k in range(count)], span, ios, peak, errs
 
-    # plain task queue
-    lock = threading.Lock()
-    t_done = [0.0] * count
-    im.task_queue_init(0)
-    tq0 = [0.0]
+    def run_queue():
+        lock = threading.Lock()
+        t_done = [0.0] * count
+        im.task_queue_init(0)
 
-    def sub_q(k, io):
-        def cb(arr, w, h, f, k=k):
-            with lock:
-                t_done[k] = time.perf_counter()
-        _, n, c = trace[k]
-        im.add_task(io, cb, "", n_cycles=c, n_hio_cycles=N_HIO, target_error=FLT_MIN,
-                    cutoff_autocorel=0.04)
+        def sub_q(k, io):
+            def cb(arr, w, h, f, k=k):
+                with lock:
+                    t_done[k] = time.perf_counter()
+            _, n, c = trace[k]
+            im.add_task(io, cb, "", n_cycles=c, n_hio_cycles=N_HIO, target_error=FLT_MIN,
+                        cutoff_autocorel=0.04)
 
-    tq0[0], ios_q = replay(sub_q, lambda: None)
-    im.task_queue_deinit()
-    lat_q = [(t_done[k] - (tq0[0] + trace[k][0])) * 1e3 for k in range(count)]
-    span_q = max(t_done) - tq0[0]
+        t0, ios = replay(sub_q, lambda: None)
+        im.task_queue_deinit()
+        lat = [(t_done[k] - (t0 + trace[k][0])) * 1e3 for k in range(count)]
+        return lat, max(t_done) - t0, ios
+
+    res_s, res_q = {}, {}
+    for graphs in (1, 0):
+        im.set_option(im.OPT_CYCLE_GRAPHS, graphs)
+        log(f"[bench] serving replay, cycle graphs {'on' if graphs else 'off'}")
+        res_s[graphs] = run_server()
+        res_q[graphs] = run_queue()
+    im.set_option(im.OPT_CYCLE_GRAPHS, 1)
+    gs = min(res_s, key=lambda g: res_s[g][1])
+    gq = min(res_q, key=lambda g: res_q[g][1])
+    lat_s, span_s, ios_s, peak, bad = res_s[gs]
+    lat_q, span_q, ios_q = res_q[gq]
     same = all(np.array_equal(a, b) for a, b in zip(ios_s, ios_q))
     out = {"requests": count, "arrival_rate_per_s": rate, "sizes": "round(exp(U[ln128, ln2048]))",
            "cycles": "5/10/20 x 20 HIO", "hbm_budget_gib": budget_gib, "peak_state_gib": round(peak / 2**30, 3),
@@ -250,6 +270,10 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200) -> d
                       "hio_iters_per_s": round(iters / span_s, 1), **_percentiles(lat_s), "errors": bad},
            "task_queue": {"recon_per_s": round(count / span_q, 2),
                           "hio_iters_per_s": round(iters / span_q, 1), **_percentiles(lat_q)},
+           "server_cycle_graphs": bool(gs), "task_queue_cycle_graphs": bool(gq),
+           "span_s_by_mode": {("graphs" if g else "eager"): {"server": round(res_s[g][1], 3),
+                                                              "task_queue": round(res_q[g][1], 3)}
+                              for g in res_s},
            "outputs_bitwise_equal": bool(same)}
     out["speedup_recon_per_s"] = round(out["server"]["recon_per_s"] / out["task_queue"]["recon_per_s"], 3)
     log(f"[bench] serving: server {out['server']}, task queue {out['task_queue']}, equal {same}")

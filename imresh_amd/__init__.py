@@ -84,6 +84,7 @@ _SIGS = {
     "imresh_memcpy_d2d": (c_int, [_P, _P, c_uint64, _P]),
     "imresh_stream_sync": (c_int, [_P]),
     "imresh_stream_create": (c_int, [POINTER(c_void_p)]),
+    "imresh_set_option": (c_int, [c_int, c_int]),
     "imresh_stream_destroy": (c_int, [_P]),
     "imresh_shrinkwrap": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), _P, POINTER(Result)]),
     "imresh_shrinkwrap_device": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), _P,
@@ -218,6 +219,13 @@ class DeviceBuffer:
 
 def sync(stream=None) -> None:
     check(_lib().imresh_stream_sync(stream), "imresh_stream_sync")
+
+
+OPT_CYCLE_GRAPHS = 1
+
+
+def set_option(option: int, value: int) -> None:
+    check(_lib().imresh_set_option(option, value), "imresh_set_option")
 
 
 def stream_create() -> int:

@@ -159,30 +159,6 @@ int work_stream(int device, hipStream_t *out)
     return IMRESH_OK;
 }
 
-// zero-padded weights rows: row c holds cycle c's 2r_c+1 taps centred in
-// 2R+1 slots, R = widest radius over the cycles
-int batch_weights(const Params &q, std::vector<float> &rows, int *R)
-{
-    std::vector<float> sig;
-    sigma_schedule(q.sigma0, q.dsigma, q.n_cycles, sig);
-    int rmax = 0;
-    std::vector<int> taps(q.n_cycles);
-    for (unsigned c = 0; c < q.n_cycles; ++c) {
-        taps[c] = gaussian_weights(sig[c], nullptr, 0);
-        if (taps[c] > 2 * kMaxBlurRadius + 1) return IMRESH_ERR_KERNEL_TOO_LARGE;
-        rmax = std::max(rmax, (taps[c] - 1) / 2);
-    }
-    rows.assign((size_t) std::max(1u, q.n_cycles) * kBlurWeightRow, 0.f);
-    for (unsigned c = 0; c < q.n_cycles; ++c) {
-        float w[2 * kMaxBlurRadius + 1];
-        gaussian_weights(sig[c], w, 2 * kMaxBlurRadius + 1);
-        const int r = (taps[c] - 1) / 2;
-        for (int k = 0; k < taps[c]; ++k) rows[(size_t) c * kBlurWeightRow + (rmax - r) + k] = w[k];
-    }
-    *R = rmax;
-    return IMRESH_OK;
-}
-
 struct Class {
     int W, H;
     std::vector<int> idx;   // positions in the caller's list
@@ -260,7 +236,7 @@ int run_batch(const imresh_image *imgs, uint32_t count, const imresh_params *par
     }
     std::vector<float> wrows;
     int R = 0;
-    IMR_CHECK(batch_weights(q, wrows, &R));
+    IMR_CHECK(padded_weights(q, wrows, &R));
 
     hipStream_t s;
     IMR_CHECK(work_stream(dev, &s));
@@ -327,8 +303,21 @@ int run_batch(const imresh_image *imgs, uint32_t count, const imresh_params *par
         if (st == IMRESH_OK) st = launch_row(ROW_INIT, k.p, ra, s);
     }
 
-    // one cycle of every class as a graph, each class on its own branch
-    if (st == IMRESH_OK && q.n_cycles > 0) {
+    // one cycle of every class as a graph, each class on its own branch;
+    // without graphs every class runs all its cycles on its own stream
+    if (st == IMRESH_OK && q.n_cycles > 0) st = prepare_blur(R);
+    if (st == IMRESH_OK && q.n_cycles > 0 && !g_use_graphs) {
+        hip(hipEventRecord(ev, s));
+        for (auto &kv : classes) {
+            Class &k = kv.second;
+            if (st != IMRESH_OK) break;
+            hip(hipStreamWaitEvent(k.buf->branch, ev, 0));
+            for (unsigned cyc = 0; cyc < q.n_cycles && st == IMRESH_OK; ++cyc)
+                st = enqueue_cycle(k, q, weights, R, k.buf->branch);
+            hip(hipEventRecord(k.buf->join, k.buf->branch));
+            hip(hipStreamWaitEvent(s, k.buf->join, 0));
+        }
+    } else if (st == IMRESH_OK && q.n_cycles > 0) {
         hip(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
         hip(hipEventRecord(ev, s));
         for (auto &kv : classes) {

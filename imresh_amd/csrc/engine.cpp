@@ -100,6 +100,7 @@ void ws_release(Workspace *w)
     if (w->weights) hipFree(w->weights);
     if (w->host_stage) hipHostFree(w->host_stage);
     if (w->ready) hipEventDestroy(w->ready);
+    if (w->gexec) hipGraphExecDestroy(w->gexec);
     delete w;
 }
 
@@ -197,35 +198,57 @@ namespace {
 
 namespace imr {
 
-// per-cycle Gaussian weights, uploaded once per parameter set (a table for
-// more cycles of the same schedule serves fewer cycles as well); the host
-// copy lives in the workspace, so the upload needs no synchronization
-int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
+// Zero-padded per-cycle weights: row c (kBlurWeightRow floats) holds cycle
+// c's 2r_c+1 taps centred in 2R+1 slots, R = widest radius over the cycles.
+// A zero tap adds exactly +0 to the reference-order sum of non-negative
+// inputs, so one blur kernel (radius R) serves every cycle bit-identically,
+// and the cycle's row is picked on the device from the completed-cycle
+// count -- the same work every cycle, which is what lets a cycle be replayed
+// as one graph.
+int padded_weights(const Params &q, std::vector<float> &rows, int *R)
 {
-    if (w->weights && w->w_sigma0 == q.sigma0 && w->w_dsigma == q.dsigma && w->w_ncycles >= q.n_cycles)
-        return IMRESH_OK;
     std::vector<float> sig;
     sigma_schedule(q.sigma0, q.dsigma, q.n_cycles, sig);
-    std::vector<float> &host = w->host_weights;
-    host.assign((size_t) std::max(1u, q.n_cycles) * 64, 0.f);
-    w->taps.assign(q.n_cycles, 0);
+    int rmax = 0;
+    std::vector<int> taps(q.n_cycles);
     for (unsigned c = 0; c < q.n_cycles; ++c) {
-        const int k = gaussian_weights(sig[c], nullptr, 0);
-        if (k > 63) return IMRESH_ERR_KERNEL_TOO_LARGE;
-        gaussian_weights(sig[c], &host[(size_t) c * 64], 64);
-        w->taps[c] = k;
+        taps[c] = gaussian_weights(sig[c], nullptr, 0);
+        if (taps[c] > 2 * kMaxBlurRadius + 1) return IMRESH_ERR_KERNEL_TOO_LARGE;
+        rmax = std::max(rmax, (taps[c] - 1) / 2);
     }
-    if (w->wcap < std::max(1u, q.n_cycles)) {
+    rows.assign((size_t) std::max(1u, q.n_cycles) * kBlurWeightRow, 0.f);
+    for (unsigned c = 0; c < q.n_cycles; ++c) {
+        float w[2 * kMaxBlurRadius + 1];
+        gaussian_weights(sig[c], w, 2 * kMaxBlurRadius + 1);
+        const int r = (taps[c] - 1) / 2;
+        for (int k = 0; k < taps[c]; ++k) rows[(size_t) c * kBlurWeightRow + (rmax - r) + k] = w[k];
+    }
+    *R = rmax;
+    return IMRESH_OK;
+}
+
+// the workspace's weights table for q (uploaded once per parameter set; the
+// host copy lives in the workspace, so the upload needs no synchronization)
+int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
+{
+    if (w->weights && w->w_sigma0 == q.sigma0 && w->w_dsigma == q.dsigma && w->w_ncycles == q.n_cycles)
+        return IMRESH_OK;
+    std::vector<float> &host = w->host_weights;
+    int R = 0;
+    IMR_CHECK(padded_weights(q, host, &R));
+    const unsigned rows = std::max(1u, q.n_cycles);
+    if (w->wcap < rows) {
         if (w->weights) {
             IMR_HIP_CHECK(hipStreamSynchronize(s));
             hipFree(w->weights);
             w->weights = nullptr;
         }
-        w->wcap = std::max(1u, q.n_cycles);
-        IMR_HIP_CHECK(hipMalloc(&w->weights, sizeof(float) * 64 * w->wcap));
+        w->wcap = rows;
+        IMR_HIP_CHECK(hipMalloc(&w->weights, sizeof(float) * kBlurWeightRow * w->wcap));
     }
     IMR_HIP_CHECK(hipMemcpyAsync(w->weights, host.data(), sizeof(float) * host.size(),
                                  hipMemcpyHostToDevice, s));
+    w->wR = R;
     w->w_sigma0 = q.sigma0;
     w->w_dsigma = q.dsigma;
     w->w_ncycles = q.n_cycles;
@@ -345,39 +368,124 @@ int ws_tune(Workspace *w, hipStream_t s) { return autotune(w->device, w->W, w->H
 int recon_begin(Workspace *w, const float *M, const Params &q, hipStream_t s)
 {
     IMR_CHECK(ensure_weights(w, q, s));
+    if (M != w->M)
+        IMR_HIP_CHECK(hipMemcpyAsync(w->M, M, sizeof(float) * w->n, hipMemcpyDeviceToDevice, s));
     IMR_HIP_CHECK(hipMemsetAsync(w->flags, 0, 64, s));
     IMR_HIP_CHECK(hipMemsetAsync(w->scal, 0, 64, s));
     Plan2D p;
     IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
-    IMR_CHECK(launch_col(COL_INIT, p, col_args(w, M), s));
+    IMR_CHECK(launch_col(COL_INIT, p, col_args(w, w->M), s));
     return launch_row(ROW_INIT, p, row_args(w, q), s);
 }
 
-int recon_cycle(Workspace *w, const float *M, const Params &q, unsigned c, hipStream_t s)
+namespace {
+
+// one shrink-wrap cycle on the workspace (the batch kernels with one image):
+// blur of |g'| with the cycle's weights row, threshold, n_hio fused HIO
+// iterations, closing inverse rows, error and early-exit flag
+int enqueue_cycle(Workspace *w, const Params &q, const Plan2D &p, hipStream_t s)
 {
-    Plan2D p;
-    IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
-    const ColArgs ca = col_args(w, M);
-    const RowArgs ra = row_args(w, q);
     BlurArgs ba{};
     ba.in = w->Bn;
     ba.out = w->B2;
-    ba.wh = ba.wv = w->weights + (size_t) c * 64;
-    ba.rh = ba.rv = (w->taps[c] - 1) / 2;
+    ba.wh = ba.wv = w->weights;
+    ba.rh = ba.rv = w->wR;
     ba.W = w->W;
     ba.H = w->H;
     ba.pmax = w->pmax;
     ba.done = w->flags;
+    ba.cycles = (const unsigned *) (w->flags + 1);
     IMR_CHECK(launch_blur(ba, s));
-    IMR_CHECK(launch_max_finalize(w->pmax, blur_grid(w->W, w->H), c == 0 ? q.cutA : q.cut, w->scal,
-                                  w->flags, s));
+    IMR_CHECK(launch_max_finalize_batch(w->pmax, blur_grid(w->W, w->H), q.cutA, q.cut, w->scal,
+                                        w->flags, 1, s));
+    const ColArgs ca = col_args(w, w->M);
+    const RowArgs ra = row_args(w, q);
     for (unsigned it = 0; it < q.n_hio; ++it) {
         IMR_CHECK(launch_row(it == 0 ? ROW_FIRST_CUT : ROW_MID, p, ra, s));
         IMR_CHECK(launch_col(COL_PROJ, p, ca, s));
     }
     IMR_CHECK(launch_row(ROW_LAST, p, ra, s));
-    return launch_err_finalize(w->perr, w->pcnt, row_grid(p), q.target, w->scal + 1, w->flags,
-                               (unsigned *) (w->flags + 1), s);
+    return launch_err_finalize_batch(w->perr, w->pcnt, row_grid(p), q.target, w->scal, w->flags, 1, s);
+}
+
+std::mutex g_cap_mtx;
+std::map<int, hipStream_t> g_cap_streams;
+
+// capture streams (a capture records, it runs nothing; the legacy default
+// stream cannot capture)
+int capture_stream(int device, hipStream_t *out)
+{
+    std::lock_guard<std::mutex> lk(g_cap_mtx);
+    auto it = g_cap_streams.find(device);
+    if (it != g_cap_streams.end()) {
+        *out = it->second;
+        return IMRESH_OK;
+    }
+    hipStream_t cs;
+    IMR_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    g_cap_streams[device] = cs;
+    *out = cs;
+    return IMRESH_OK;
+}
+
+bool same_key(const Workspace *w, const Params &q, const Plan2D &p)
+{
+    return w->gexec != nullptr && w->g_nhio == q.n_hio && w->g_beta == q.beta && w->g_cutA == q.cutA &&
+           w->g_cut == q.cut && w->g_target == q.target && w->g_R == w->wR && w->g_weights == w->weights &&
+           w->g_rowVar == p.rowVar && w->g_colVar == p.colVar;
+}
+
+// the cycle as a graph, captured once per workspace and parameter set
+int cycle_graph(Workspace *w, const Params &q, const Plan2D &p)
+{
+    if (same_key(w, q, p)) return IMRESH_OK;
+    if (w->gexec) {
+        hipGraphExecDestroy(w->gexec);
+        w->gexec = nullptr;
+    }
+    IMR_CHECK(prepare_blur(w->wR));
+    hipStream_t cs;
+    IMR_CHECK(capture_stream(w->device, &cs));
+    hipGraph_t g = nullptr;
+    std::lock_guard<std::mutex> lk(g_cap_mtx);   // one capture per stream at a time
+    IMR_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    const int st = enqueue_cycle(w, q, p, cs);
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    if (st != IMRESH_OK) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
+    IMR_HIP_CHECK(e);
+    const hipError_t ei = hipGraphInstantiate(&w->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    IMR_HIP_CHECK(ei);
+    w->g_nhio = q.n_hio;
+    w->g_beta = q.beta;
+    w->g_cutA = q.cutA;
+    w->g_cut = q.cut;
+    w->g_target = q.target;
+    w->g_R = w->wR;
+    w->g_weights = w->weights;
+    w->g_rowVar = p.rowVar;
+    w->g_colVar = p.colVar;
+    return IMRESH_OK;
+}
+
+}  // namespace
+
+bool g_use_graphs = true;
+
+int recon_cycle(Workspace *w, const float *M, const Params &q, unsigned c, hipStream_t s)
+{
+    (void) M;
+    (void) c;   // the cycle's weights and cut-off are chosen on the device
+    Plan2D p;
+    IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
+    if (g_use_graphs) {
+        IMR_CHECK(cycle_graph(w, q, p));
+        return from_hip(hipGraphLaunch(w->gexec, s));
+    }
+    return enqueue_cycle(w, q, p, s);
 }
 
 int recon_end(Workspace *w, float *io_dev, hipStream_t s)
@@ -624,6 +732,14 @@ int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream)
 }
 
 int imresh_stream_sync(void *stream) { return from_hip(hipStreamSynchronize(S(stream))); }
+
+int imresh_set_option(int option, int value)
+{
+    switch (option) {
+    case IMRESH_OPT_CYCLE_GRAPHS: g_use_graphs = value != 0; return IMRESH_OK;
+    default: return IMRESH_ERR_INVALID;
+    }
+}
 
 int imresh_stream_create(void **stream)
 {
@@ -875,7 +991,7 @@ int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms,
     BlurArgs ba{};
     ba.in = w->Bn; ba.out = w->B2; ba.wh = ba.wv = w->weights;
     ba.W = W; ba.H = H; ba.pmax = w->pmax;
-    if (st == IMRESH_OK) ba.rh = ba.rv = (w->taps[0] - 1) / 2;
+    if (st == IMRESH_OK) ba.rh = ba.rv = w->wR;
     // each kernel back to back in batches of 8 launches on a freshly filled
     // state (fill values are below 1; the unnormalized row transforms grow
     // them by about n per launch, 2048^8 stays finite)

@@ -34,15 +34,21 @@ struct Workspace {
     unsigned *pcnt = nullptr;
     float *scal = nullptr;      // [0] threshold, [1..3] error, total, nmasked
     int *flags = nullptr;       // [0] done, [1] cycles
-    float *weights = nullptr;   // [cycles][64]
+    float *weights = nullptr;   // [cycles][kBlurWeightRow], zero-padded to radius wR
     unsigned wcap = 0;
-    std::vector<int> taps;
+    int wR = 0;
     float w_sigma0 = -1.f, w_dsigma = -1.f;
     unsigned w_ncycles = 0;
     std::vector<float> host_weights;   // host copy of the weights table
     float *host_stage = nullptr;  // pinned, 64 floats + readback
     hipEvent_t ready = nullptr;   // recorded when the last user's work is queued
     bool pending = false;
+    // the captured cycle graph and the parameters it was captured with
+    hipGraphExec_t gexec = nullptr;
+    unsigned g_nhio = 0;
+    float g_beta = 0, g_cutA = 0, g_cut = 0, g_target = 0;
+    int g_R = -1, g_rowVar = -1, g_colVar = -1;
+    const float *g_weights = nullptr;
 };
 
 int ws_alloc(int device, int W, int H, Workspace **out);
@@ -50,10 +56,17 @@ void ws_release(Workspace *w);
 // device bytes of a workspace (for HBM budgets)
 size_t ws_bytes(int W, int H);
 int ensure_weights(Workspace *w, const Params &q, hipStream_t s);
+// zero-padded per-cycle weights rows (kBlurWeightRow floats each) and their
+// common radius
+int padded_weights(const Params &q, std::vector<float> &rows, int *R);
+// cycles run as captured graphs (default) or as eager launches
+extern bool g_use_graphs;
 
 // one reconstruction in steps (all asynchronous on s): begin = weights,
-// state reset, initial inverse transform of the modulus M (device);
-// cycle c = blur, threshold, n_hio fused iterations, error / early exit;
+// modulus M (device) staged into the workspace, state reset, initial
+// inverse transform; cycle = blur, threshold, n_hio fused iterations,
+// error / early exit (one graph replay; the cycle index is kept on the
+// device);
 // end = Re(g') -> io_dev.  stage_results copies the scalars to the pinned
 // host stage, read_staged decodes them once that copy has completed.
 // picks the size's kernel variants on first use; it overwrites the

@@ -146,6 +146,8 @@ IMR_DECLARE_GROUP(4)
 int row_grid(const Plan2D &p);
 int blur_grid(int W, int H);
 int launch_blur(const BlurArgs &a, hipStream_t s);
+// sets the blur kernel's LDS attribute for radius R (before a graph capture)
+int prepare_blur(int R);
 int launch_blur_h_inplace(float *d, int W, int H, const float *w, int r, hipStream_t s);
 int launch_max_finalize_batch(const float *pmax, int np, float cutA, float cut, float *scal,
                               int *flags, int batch, hipStream_t s);

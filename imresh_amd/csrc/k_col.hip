@@ -31,11 +31,19 @@ enum ColKernel { CK_FWD, CK_INV, CK_PROJ };
 // LDS one after the other (half the LDS of a complex tile); PF (CK_PROJ
 // only): the measured modulus is read right after the column gather instead
 // of at the junction.
-template <class P, int C, bool SPLIT, int K, bool PF = false, bool BL = false>
+template <class P, int C, bool SPLIT, int K, bool PF = false, bool BL = false, int STG = 0>
 __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 {
     const int b = blockIdx.y;   // image of a batch
     if (a.done != nullptr && a.done[b * kFlagStride]) return;
+    if constexpr (STG > 0) {
+        // phase offset: the second half of the grid shares CUs with the first
+        // half; starting it STG x ~3.4 us later lets the loads of one overlap
+        // the arithmetic of the other instead of every workgroup loading,
+        // computing and storing in lockstep
+        if (blockIdx.x >= (gridDim.x >> 1))
+            for (int i = 0; i < STG; ++i) __builtin_amdgcn_s_sleep(127);
+    }
     extern __shared__ cf smem[];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
@@ -92,7 +100,7 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 template <class P, int C, bool SPLIT>
 constexpr size_t col_lds() { return (SPLIT ? sizeof(float) : sizeof(cf)) * (size_t) lds_elems(C * P::n); }
 
-template <class P, int C, bool SPLIT, bool PF = false, bool BL = false>
+template <class P, int C, bool SPLIT, bool PF = false, bool BL = false, int STG = 0>
 int launch_variant(int mode, ColArgs a, hipStream_t s)
 {
     static_assert(C * P::T <= 1024, "column workgroup too large");
@@ -104,7 +112,7 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
             hipError_t e = hipSuccess;
             for (const void *f : {(const void *) k_col<P, C, SPLIT, CK_FWD, false, BL>,
                                   (const void *) k_col<P, C, SPLIT, CK_INV, false, BL>,
-                                  (const void *) k_col<P, C, SPLIT, CK_PROJ, PF, BL>})
+                                  (const void *) k_col<P, C, SPLIT, CK_PROJ, PF, BL, STG>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -128,7 +136,7 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
         break;
     case COL_PROJ:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ, PF, BL>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ, PF, BL, STG>), grid, block, lds, s, a);
         break;
     default: return IMRESH_ERR_INVALID;
     }
@@ -136,25 +144,31 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
 }
 
 // Column variants of a length (index: tile shape, then +count/2 for the
-// prefetching CK_PROJ): shapes 0 = E 16 with the plan's default tile; for
+// prefetching CK_PROJ): shape 0 = E 16 with the plan's default tile; for
 // long columns also 1 = E 32 (half the threads per column, 512-thread
-// groups) and 2 = E 32 with 16-column split-LDS tiles (128-byte row
-// segments).
+// groups), 2 = E 16 with 4-column tiles (two workgroups per CU) and 3, 4 =
+// the same with the second half of the grid started ~3.4 / ~6.8 us late.
+template <int N>
+constexpr bool blue_length() { return N >= 64 && N <= 2 * kMaxLine && (N & (N - 1)) == 0; }
+
 template <int N, int... Rs>
 constexpr int col_shape_count()
 {
-    if constexpr (N >= 1024) {
-        using P32 = PlanE<32, N, Rs...>;
-        return (16 * P32::T <= 1024 && col_lds<P32, 16, true>() <= kLdsBudget) ? 3 : 2;
-    }
+    using P16 = PlanE<16, N, Rs...>;
+    if constexpr (N >= 1024 && P16::C > 4) return 5;
+    else if constexpr (N >= 1024) return 2;
     return 1;
 }
 
 template <int N, int... Rs>
 constexpr int col_variant_count() { return 2 * col_shape_count<N, Rs...>(); }
 
-template <int N>
-constexpr bool blue_length() { return N >= 64 && N <= 2 * kMaxLine && (N & (N - 1)) == 0; }
+template <class P, int C, int STG = 0>
+int launch_pf(bool pf, int mode, ColArgs a, hipStream_t s)
+{
+    return pf ? launch_variant<P, C, false, true, false, STG>(mode, a, s)
+              : launch_variant<P, C, false, false, false, STG>(mode, a, s);
+}
 
 template <int N, int... Rs>
 int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
@@ -167,16 +181,14 @@ int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
     const int shape = pf ? var - ns : var;
     if constexpr (N >= 1024) {
         using P32 = PlanE<32, N, Rs...>;
-        if (shape == 1)
-            return pf ? launch_variant<P32, P32::C, false, true>(mode, a, s)
-                      : launch_variant<P32, P32::C, false>(mode, a, s);
-        if constexpr (ns == 3)
-            if (shape == 2)
-                return pf ? launch_variant<P32, 16, true, true>(mode, a, s)
-                          : launch_variant<P32, 16, true>(mode, a, s);
+        if (shape == 1) return launch_pf<P32, P32::C>(pf, mode, a, s);
+        if constexpr (ns == 5) {
+            if (shape == 2) return launch_pf<P16, 4>(pf, mode, a, s);
+            if (shape == 3) return launch_pf<P16, 4, 1>(pf, mode, a, s);
+            if (shape == 4) return launch_pf<P16, 4, 2>(pf, mode, a, s);
+        }
     }
-    return pf ? launch_variant<P16, P16::C, false, true>(mode, a, s)
-              : launch_variant<P16, P16::C, false>(mode, a, s);
+    return launch_pf<P16, P16::C>(pf, mode, a, s);
 }
 
 template <int G, int N, int... Rs>

@@ -376,6 +376,33 @@ static int launch_blur_r(const BlurArgs &a, hipStream_t s)
     return from_hip(hipGetLastError());
 }
 
+template <int R>
+static int prepare_blur_r()
+{
+    constexpr int AWP = (kBlurTX + 2 * R + 3) & ~3, AH = kBlurTY + 2 * R;
+    constexpr size_t lds = sizeof(float) * (size_t) AWP * AH;
+    if constexpr (lds > 64 * 1024) {
+        static const hipError_t attr = hipFuncSetAttribute(
+            (const void *) k_blur2d<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+        return from_hip(attr);
+    }
+    return IMRESH_OK;
+}
+
+template <int... Rs>
+static int prepare_blur_dispatch(int r, std::integer_sequence<int, Rs...>)
+{
+    int st = IMRESH_ERR_INVALID;
+    ((r == Rs ? (st = prepare_blur_r<Rs>(), true) : false) || ...);
+    return st;
+}
+
+int prepare_blur(int R)
+{
+    if (R < 0 || R > kMaxBlurRadius) return IMRESH_ERR_INVALID;
+    return prepare_blur_dispatch(R, std::make_integer_sequence<int, kMaxBlurRadius + 1>{});
+}
+
 template <int... Rs>
 static int launch_blur_dispatch(int r, const BlurArgs &a, hipStream_t s,
                                 std::integer_sequence<int, Rs...>)

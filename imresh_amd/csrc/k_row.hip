@@ -32,13 +32,18 @@ enum RowKernel { RK_FWD, RK_INV, RK_INVOUT, RK_FIRST, RK_MID };
 // NL rows per workgroup; PF (RK_MID only): gPrev and the mask are read right
 // after the row gather instead of at the junction; BL: Bluestein kernels of
 // row length a.len with P the convolution plan
-template <class P, int NL, int K, bool PF = false, bool BL = false>
+template <class P, int NL, int K, bool PF = false, bool BL = false, int STG = 0>
 __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
 {
     // image b of a batch (grid.y): its planes start b * istride elements in,
     // its scalars at b * kScalStride / kFlagStride
     const int b = blockIdx.y;
     if (a.done != nullptr && a.done[b * kFlagStride]) return;
+    if constexpr (STG > 0) {
+        // phase offset of the second half of the grid (see k_col)
+        if (blockIdx.x >= (gridDim.x >> 1))
+            for (int i = 0; i < STG; ++i) __builtin_amdgcn_s_sleep(127);
+    }
     extern __shared__ cf smem[];
     constexpr int T = P::T;
     const int n = BL ? a.len : P::n;
@@ -139,7 +144,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     }
 }
 
-template <class P, int NL, bool PF = false, bool BL = false>
+template <class P, int NL, bool PF = false, bool BL = false, int STG = 0>
 int launch_variant(int mode, RowArgs a, hipStream_t s)
 {
     static_assert(NL * P::T <= 1024, "row workgroup too large");
@@ -153,7 +158,7 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
                                   (const void *) k_row<P, NL, RK_INV, false, BL>,
                                   (const void *) k_row<P, NL, RK_INVOUT, false, BL>,
                                   (const void *) k_row<P, NL, RK_FIRST, false, BL>,
-                                  (const void *) k_row<P, NL, RK_MID, PF, BL>})
+                                  (const void *) k_row<P, NL, RK_MID, PF, BL, STG>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
             return e;
@@ -183,7 +188,7 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         a.flags = 0;
         hipLaunchKernelGGL((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
         break;
-    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID, PF, BL>), grid, block, lds, s, a); break;
+    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID, PF, BL, STG>), grid, block, lds, s, a); break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());
@@ -191,15 +196,15 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
 
 // Row variants of a length (index: workgroup shape, then +count/2 for the
 // prefetching RK_MID): shapes 0 = E 16 with ~256-thread groups; for long
-// rows also 1 = E 32 (half the threads per row) and 2 = E 16 with twice the
-// rows.
+// rows also 1 = E 32 (half the threads per row), 2 = E 16 with twice the
+// rows and 3 = shape 0 with the second half of the grid started ~3.4 us late.
 template <int N, int... Rs>
 constexpr int row_shape_count()
 {
     if constexpr (N >= 1024) {
         using P16 = PlanE<16, N, Rs...>;
         return (2 * P16::L * P16::T <= 1024 &&
-                sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 3 : 2;
+                sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 4 : 2;
     }
     return 1;
 }
@@ -210,6 +215,13 @@ constexpr int row_variant_count() { return 2 * row_shape_count<N, Rs...>(); }
 // Bluestein kernels exist for the power-of-two convolution lengths
 template <int N>
 constexpr bool blue_length() { return N >= 64 && N <= 2 * kMaxLine && (N & (N - 1)) == 0; }
+
+template <class P, int NL, int STG = 0>
+int launch_pf(bool pf, int mode, RowArgs a, hipStream_t s)
+{
+    return pf ? launch_variant<P, NL, true, false, STG>(mode, a, s)
+              : launch_variant<P, NL, false, false, STG>(mode, a, s);
+}
 
 template <int N, int... Rs>
 int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
@@ -222,14 +234,13 @@ int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
     const int shape = pf ? var - ns : var;
     if constexpr (N >= 1024) {
         using P32 = PlanE<32, N, Rs...>;
-        if (shape == 1)
-            return pf ? launch_variant<P32, P32::L, true>(mode, a, s) : launch_variant<P32, P32::L>(mode, a, s);
-        if constexpr (ns == 3)
-            if (shape == 2)
-                return pf ? launch_variant<P16, 2 * P16::L, true>(mode, a, s)
-                          : launch_variant<P16, 2 * P16::L>(mode, a, s);
+        if (shape == 1) return launch_pf<P32, P32::L>(pf, mode, a, s);
+        if constexpr (ns == 4) {
+            if (shape == 2) return launch_pf<P16, 2 * P16::L>(pf, mode, a, s);
+            if (shape == 3) return launch_pf<P16, P16::L, 1>(pf, mode, a, s);
+        }
     }
-    return pf ? launch_variant<P16, P16::L, true>(mode, a, s) : launch_variant<P16, P16::L>(mode, a, s);
+    return launch_pf<P16, P16::L>(pf, mode, a, s);
 }
 
 // rows per workgroup of a variant (the host needs it for the partial sums)

@@ -302,7 +302,7 @@ int imresh_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_
     Server &s = h->s;
     s.device = dev;
     s.budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t) 200 << 30;
-    s.max_active = max_active ? max_active : 64;
+    s.max_active = max_active ? max_active : 8;
     const uint32_t nl = lanes ? lanes : 8;
     for (uint32_t i = 0; i < nl; ++i) {
         hipStream_t st = nullptr;

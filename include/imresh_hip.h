@@ -87,6 +87,12 @@ int imresh_memcpy_d2h(void *host, const void *dev, uint64_t bytes, void *stream)
 /* asynchronous device-to-device copy on `stream` */
 int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream);
 int imresh_stream_sync(void *stream);
+/* Library options (process-wide).  IMRESH_OPT_CYCLE_GRAPHS: 1 (default)
+ * replays each shrink-wrap cycle as a captured HIP graph, 0 launches its
+ * kernels one by one; both compute the same bits. */
+enum { IMRESH_OPT_CYCLE_GRAPHS = 1 };
+int imresh_set_option(int option, int value);
+
 /* A non-blocking HIP stream at the default priority (for callers without a
  * HIP runtime of their own, e.g. to overlap reconstructions), and its
  * destruction. */
@@ -157,7 +163,7 @@ typedef void (*imresh_server_callback)(void *user, const imresh_server_completio
 
 /* Continuous batching on the current device: requests of any supported
  * size are admitted in arrival order while fewer than max_active run and
- * their device state fits hbm_budget_bytes (0: 200 GiB; max_active 0: 64;
+ * their device state fits hbm_budget_bytes (0: 200 GiB; max_active 0: 8;
  * lanes = HIP streams, 0: 8); every admitted request advances cycle by cycle
  * with its state resident on the device, interleaved with the others.  Each
  * result equals imresh_shrinkwrap on the same input. */

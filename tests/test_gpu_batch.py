@@ -149,3 +149,21 @@ def test_batch_repeated_calls_reuse_buffers():
         im.shrink_wrap_batch(ios, _params(**KW))
         for M, io in zip(Ms, ios):
             assert np.array_equal(io, _single(M, **KW)[0])
+
+
+def test_cycle_graphs_equal_eager_launches():
+    """Each cycle replays a captured graph (default); with graphs off the
+    same kernels are launched one by one.  Several reconstructions in a row
+    (replays after the first, eager work in between) must agree bitwise."""
+    Ms = _moduli([(128, 128), (97, 97), (128, 128)], seed=21)
+    kw = dict(n_cycles=4, n_hio_cycles=3, target_error=FLT_MIN)
+    try:
+        im.set_option(im.OPT_CYCLE_GRAPHS, 0)
+        eager = [_single(M, **kw) for M in Ms]
+    finally:
+        im.set_option(im.OPT_CYCLE_GRAPHS, 1)
+    for _ in range(2):
+        for M, (x, r) in zip(Ms, eager):
+            y, s = _single(M, **kw)
+            assert np.array_equal(x, y)
+            assert r.last_error == s.last_error and r.cycles_run == s.cycles_run
