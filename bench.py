@@ -10,9 +10,11 @@ in-place io buffer.  Early exit is disabled with target error = FLT_MIN, as
 the reference's own test does.
 
 Prints ONE JSON line (the benchmark contract) with two extra objects:
-  roofline      the dominant fused kernel's measured duration (HIP events on
-                the launch stream, imresh_profile_kernels) against its
-                compulsory HBM bytes per launch and the 8 TB/s HBM peak;
+  roofline      the dominant fused kernel's measured duration (kernel-exact
+                HIP events recorded around each launch on the launch stream
+                by imresh_profile_kernels, row and column passes alternating
+                as in a reconstruction) against its compulsory HBM bytes per
+                launch and the 8 TB/s HBM peak;
                 `traffic` from rocprofv3 FETCH_SIZE/WRITE_SIZE summaries in
                 profiles/ when present
   cpu_baseline  the CPU oracle (C restatement of the reference CPU path,
@@ -332,7 +334,7 @@ def main() -> int:
         log("[bench] non-finite reconstruction")
         return 1
 
-    log("[bench] per-kernel durations (HIP events)")
+    log("[bench] per-kernel durations (kernel-exact HIP events)")
     ms = (im.ctypes.c_float * 4)()
     im.check(L.imresh_profile_kernels(W, H, 50, ms, None), "profile")
     per = dict(zip(KERNEL_NAMES, [float(x) for x in ms]))

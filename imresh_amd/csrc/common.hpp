@@ -2,6 +2,7 @@
 // library: status codes, launch descriptors and the line-FFT plan layout.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -35,3 +36,23 @@ inline int from_hip(hipError_t e)
     return IMRESH_ERR_HIP;
 }
 }  // namespace imr
+
+namespace imr {
+// Kernel-exact timing for the profiler entry point: while `start` is set on
+// the calling thread, IMR_LAUNCH records the two events around the kernel's
+// own execution (hipExtLaunchKernelGGL), which is what rocprofv3's kernel
+// trace measures; otherwise it is a plain launch.
+struct ProfEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local ProfEvents g_prof;
+}  // namespace imr
+
+#define IMR_LAUNCH(K, grid, block, lds, s, ...)                                                \
+    do {                                                                                       \
+        if (imr::g_prof.start != nullptr)                                                      \
+            hipExtLaunchKernelGGL(K, grid, block, lds, s, imr::g_prof.start, imr::g_prof.stop, \
+                                  0, __VA_ARGS__);                                             \
+        else                                                                                   \
+            hipLaunchKernelGGL(K, grid, block, lds, s, __VA_ARGS__);                           \
+    } while (0)

@@ -975,54 +975,63 @@ int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms,
     Params q;
     normalize(nullptr, q);
     q.n_cycles = 1;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
     double acc[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
     int st = autotune(dev, W, H, s, w);
     Plan2D p;
     if (st == IMRESH_OK) st = get_plan2d(dev, W, H, &p);
     if (st == IMRESH_OK) st = ensure_weights(w, q, s);
-    if (st == IMRESH_OK) st = from_hip(hipEventCreate(&e0));
-    if (st == IMRESH_OK) st = from_hip(hipEventCreate(&e1));
-    RowArgs ra{};
-    ra.X = w->X; ra.G = w->G; ra.mask = w->mask; ra.Bn = w->Bn; ra.B2 = w->B2;
-    ra.thr = w->scal; ra.beta = 0.9f; ra.perr = w->perr; ra.pcnt = w->pcnt;
-    ColArgs ca{};
-    ca.X = w->X; ca.M = w->M;
+    std::vector<hipEvent_t> ev;
+    const RowArgs ra = row_args(w, q);
+    const ColArgs ca = col_args(w, w->M);
     BlurArgs ba{};
     ba.in = w->Bn; ba.out = w->B2; ba.wh = ba.wv = w->weights;
-    ba.W = W; ba.H = H; ba.pmax = w->pmax;
-    if (st == IMRESH_OK) ba.rh = ba.rv = w->wR;
-    // each kernel back to back in batches of 8 launches on a freshly filled
-    // state (fill values are below 1; the unnormalized row transforms grow
-    // them by about n per launch, 2048^8 stays finite)
-    constexpr int batch = 8;
-    const int nb = (int) ((reps + batch - 1) / batch);
-    for (int k = 0; k < 4 && st == IMRESH_OK; ++k) {
-        for (int b = -1; b < nb && st == IMRESH_OK; ++b) {   // batch -1 warms up
-            st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, W, s);
-            if (st != IMRESH_OK) break;
-            hipEventRecord(e0, s);
-            for (int r = 0; r < batch && st == IMRESH_OK; ++r) {
-                switch (k) {
-                case 0: st = launch_row(ROW_MID, p, ra, s); break;
-                case 1: st = launch_col(COL_PROJ, p, ca, s); break;
-                case 2: st = launch_blur(ba, s); break;
-                default: st = launch_row(ROW_LAST, p, ra, s); break;
-                }
-            }
-            hipEventRecord(e1, s);
-            if (st == IMRESH_OK) st = from_hip(hipEventSynchronize(e1));
-            float ms = 0.f;
-            if (st == IMRESH_OK) hipEventElapsedTime(&ms, e0, e1);
-            if (b >= 0) acc[k] += ms;
+    ba.W = W; ba.H = H; ba.pmax = w->pmax; ba.rh = ba.rv = w->wR;
+    // each launch timed on its own execution (kernel-exact events): the
+    // fused HIO iteration as it runs in a reconstruction (row pass, column
+    // pass alternating on a filled state), then the cycle-boundary kernels
+    auto timed = [&](int k, auto launch) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (st == IMRESH_OK) st = from_hip(hipEventCreate(&a));
+        if (st == IMRESH_OK) st = from_hip(hipEventCreate(&b));
+        if (st != IMRESH_OK) return;
+        g_prof.start = a;
+        g_prof.stop = b;
+        st = launch();
+        g_prof = ProfEvents{};
+        ev.push_back(a);
+        ev.push_back(b);
+        ev.push_back((hipEvent_t) (intptr_t) k);
+    };
+    const int iters = (int) std::max(4u, reps);
+    for (int round = 0; round < 2 && st == IMRESH_OK; ++round) {   // round 0 warms up
+        ev.clear();
+        st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, W, s);
+        for (int i = 0; i < iters && st == IMRESH_OK; ++i) {
+            timed(0, [&] { return launch_row(ROW_MID, p, ra, s); });
+            timed(1, [&] { return launch_col(COL_PROJ, p, ca, s); });
         }
+        for (int i = 0; i < iters && st == IMRESH_OK; ++i) {
+            timed(2, [&] { return launch_blur(ba, s); });
+            timed(3, [&] { return launch_row(ROW_LAST, p, ra, s); });
+        }
+        if (st == IMRESH_OK) st = from_hip(hipStreamSynchronize(s));
+        for (size_t e = 0; e + 2 < ev.size(); e += 3) {
+            const int k = (int) (intptr_t) ev[e + 2];
+            float ms = 0.f;
+            if (st == IMRESH_OK && round == 1 && hipEventElapsedTime(&ms, ev[e], ev[e + 1]) == hipSuccess) {
+                acc[k] += ms;
+                cnt[k] += 1;
+            }
+            hipEventDestroy(ev[e]);
+            hipEventDestroy(ev[e + 1]);
+        }
+        ev.clear();
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
     hipStreamSynchronize(s);
     ws_return(w, s);
     if (st == IMRESH_OK)
-        for (int k = 0; k < 4; ++k) out_ms[k] = (float) (acc[k] / ((double) nb * batch));
+        for (int k = 0; k < 4; ++k) out_ms[k] = cnt[k] ? (float) (acc[k] / cnt[k]) : 0.f;
     return st;
 }
 

@@ -122,21 +122,21 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
     switch (mode) {
     case COL_FWD:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INV:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INIT:
         a.flags = kColRealIn;
         a.in = a.X;   // unused with kColRealIn; never a null pointer in a kernel
         a.out = a.X;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
         break;
     case COL_PROJ:
         a.flags = 0;
-        hipLaunchKernelGGL((k_col<P, C, SPLIT, CK_PROJ, PF, BL, STG>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_col<P, C, SPLIT, CK_PROJ, PF, BL, STG>), grid, block, lds, s, a);
         break;
     default: return IMRESH_ERR_INVALID;
     }

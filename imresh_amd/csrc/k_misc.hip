@@ -372,7 +372,7 @@ static int launch_blur_r(const BlurArgs &a, hipStream_t s)
     }
     const dim3 grid((a.W + kBlurTX - 1) / kBlurTX, (a.H + kBlurTY - 1) / kBlurTY,
                     a.batch > 1 ? a.batch : 1);
-    hipLaunchKernelGGL(k_blur2d<R>, grid, dim3(256), lds, s, a);
+    IMR_LAUNCH(k_blur2d<R>, grid, dim3(256), lds, s, a);
     return from_hip(hipGetLastError());
 }
 

@@ -166,29 +166,29 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         if (attr != hipSuccess) return from_hip(attr);
     }
     switch (mode) {
-    case ROW_FWD: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
-    case ROW_INV: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_INV, false, BL>), grid, block, lds, s, a); break;
+    case ROW_FWD: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
+    case ROW_INV: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_INV, false, BL>), grid, block, lds, s, a); break;
     case ROW_INIT:
         a.flags = kRowWriteG | kRowWriteBn;
-        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_LAST:
         a.flags = kRowWriteBn | kRowError;
-        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_LAST_PLAIN:
         a.flags = 0;
-        hipLaunchKernelGGL((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_FIRST_CUT:
         a.flags = kRowCut;
-        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_FIRST_MASK:
         a.flags = 0;
-        hipLaunchKernelGGL((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
         break;
-    case ROW_MID: a.flags = 0; hipLaunchKernelGGL((k_row<P, NL, RK_MID, PF, BL, STG>), grid, block, lds, s, a); break;
+    case ROW_MID: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_MID, PF, BL, STG>), grid, block, lds, s, a); break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());

@@ -16,6 +16,8 @@
 
 namespace imr {
 
+thread_local ProfEvents g_prof;
+
 namespace {
 
 std::recursive_mutex g_plan_mtx;   // get_line_plan recurses for Bluestein lengths

@@ -6,9 +6,10 @@ launches, in this order on the default stream:
   1. a plain 128 MiB device-to-device copy (hipMemcpyAsync, 4x the aggregate
      L2), the calibration kernel of known traffic (128 MiB read + 128 MiB
      written) for FETCH_SIZE / WRITE_SIZE;
-  2. imresh_profile_kernels: each fused kernel of one HIO iteration at
-     size x size back to back (row pass, column pass, blur, closing row pass),
-     and prints their HIP-event durations.
+  2. imresh_profile_kernels: the fused HIO iteration as a reconstruction runs
+     it (row pass and column pass alternating), then the cycle-boundary
+     kernels (blur, closing row pass), each launch timed on its own execution
+     with kernel-exact HIP events, and prints the mean durations.
 """
 from __future__ import annotations
 
