@@ -97,7 +97,7 @@ def _gen_batch_input(args):
                         index=20_000_000 + i)[1]
 
 
-def generate_inputs(batch_count: int, serve_count: int):
+def generate_inputs(batch_count: int, serve_count: int, light_count: int = 150, light_rate: float = 50.0):
     """All synthetic inputs, generated in worker processes BEFORE this
     process touches the GPU (workers are forked; a process that has
     initialized the GPU must not fork helpers)."""
@@ -105,13 +105,16 @@ def generate_inputs(batch_count: int, serve_count: int):
     from concurrent.futures import ProcessPoolExecutor
     sizes = syn.batch_sizes(batch_count) if batch_count else []
     trace = syn.serve_trace(serve_count) if serve_count else []
-    log(f"[bench] generating {len(sizes)} batch and {len(trace)} serving inputs")
+    light = syn.serve_trace(light_count, rate=light_rate, index=1) if serve_count else []
+    log(f"[bench] generating {len(sizes)} batch and {len(trace)} + {len(light)} serving inputs")
     with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 4),
                              mp_context=mp.get_context("fork")) as ex:
         bmods = list(ex.map(_gen_batch_input, list(enumerate(sizes)), chunksize=8))
         smods = list(ex.map(_gen_serve_input, [(i, n) for i, (_, n, _) in enumerate(trace)],
                             chunksize=4))
-    return (sizes, bmods), (trace, smods)
+        lmods = list(ex.map(_gen_serve_input, [(100_000 + i, n) for i, (_, n, _) in enumerate(light)],
+                            chunksize=4))
+    return (sizes, bmods), (trace, smods), (light, lmods)
 
 
 def batch_bench(L, sizes, mods, reps: int = 2, plain_streams: int = 3) -> dict:
@@ -196,7 +199,7 @@ def _percentiles(lat):
             "p99.9_ms": round(float(np.percentile(a, 99.9)), 2), "max_ms": round(float(a[-1]), 2)}
 
 
-def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200) -> dict:
+def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, modes=(1, 0)) -> dict:
     """Serving workload: `count` requests of synthetic.serve_trace (Poisson
     arrivals at `rate`/s, sizes log-uniform in [128, 2048], 5/10/20 cycles x
     20 HIO), host buffers as the reference API takes them.  The same trace is
@@ -255,9 +258,9 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200) -> d
         return lat, max(t_done) - t0, ios
 
     res_s, res_q = {}, {}
-    for graphs in (1, 0):
+    for graphs in modes:
         im.set_option(im.OPT_CYCLE_GRAPHS, graphs)
-        log(f"[bench] serving replay, cycle graphs {'on' if graphs else 'off'}")
+        log(f"[bench] serving replay at {rate:g}/s, cycle graphs {'on' if graphs else 'off'}")
         res_s[graphs] = run_server()
         res_q[graphs] = run_queue()
     im.set_option(im.OPT_CYCLE_GRAPHS, 1)
@@ -282,6 +285,38 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200) -> d
     return out
 
 
+def large_bench(L, M8, steps: int = 2) -> dict:
+    """The large single image: 8192 x 8192 (512 shapes in the central 4096^2,
+    1e11 photons, sigma0 = 6.0: 35-tap blur), 20 x 20, resident in HBM."""
+    n = M8.shape[0]
+    d_src = im.DeviceBuffer.from_array(M8)
+    d_io = im.DeviceBuffer(M8.nbytes)
+    p = im.default_params(n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO, sigma0=6.0)
+
+    def step():
+        im.check(L.imresh_memcpy_d2d(d_io.ptr, d_src.ptr, M8.nbytes, None), "d2d")
+        im.check(L.imresh_shrinkwrap_device(d_io.ptr, n, n, p, None, None), "shrinkwrap 8192")
+
+    step()   # warm-up: tuning, weights, graph capture
+    im.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    im.sync()
+    dt = (time.perf_counter() - t0) / steps
+    out = d_io.download(np.float32, M8.shape)
+    ms = (im.ctypes.c_float * 4)()
+    im.check(L.imresh_profile_kernels(n, n, 8, ms, None), "profile 8192")
+    per = dict(zip(KERNEL_NAMES, [float(x) for x in ms]))
+    kernels = {k: {"ms": round(per[k], 4),
+                   "GB/s": round(KERNEL_BYTES[k] * n * n / (per[k] * 1e-3) / 1e9, 1)} for k in KERNEL_NAMES}
+    return {"workload": "shrink-wrap 8192x8192 fp32, 20 cycles x 20 HIO, sigma0 6.0 (35 taps), "
+                        "512 shapes in central 4096^2, 1e11 photons",
+            "hio_iters_per_s": round(N_CYCLES * N_HIO / dt, 1), "ms_per_reconstruction": round(dt * 1e3, 2),
+            "finite": bool(np.all(np.isfinite(out))), "kernels": kernels,
+            "iteration_alg_GBps_80B": round(80 * n * n * N_CYCLES * N_HIO / dt / 1e9, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +326,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-count", type=int, default=300)
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-fft-compare", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -300,8 +337,12 @@ def main() -> int:
 
     log("[bench] generating the 2048x2048 workload")
     obj, M = syn.config_2048(0)
-    (bsizes, bmods), (trace, smods) = generate_inputs(0 if args.no_batch else args.batch_count,
-                                                      0 if args.no_serve else args.serve_count)
+    M8 = None
+    if not args.no_large:
+        log("[bench] generating the 8192x8192 workload")
+        M8 = syn.config_8192(0)[1]
+    (bsizes, bmods), (trace, smods), (light, lmods) = generate_inputs(
+        0 if args.no_batch else args.batch_count, 0 if args.no_serve else args.serve_count)
     d_src = im.DeviceBuffer.from_array(M)
     d_io = im.DeviceBuffer(M.nbytes)
     L = im.lib()
@@ -363,6 +404,20 @@ def main() -> int:
     serve = None
     if not args.no_serve:
         serve = serve_bench(L, trace, smods)
+        log("[bench] serving below capacity (50 requests/s)")
+        serve["below_capacity"] = serve_bench(L, light, lmods, rate=50.0, modes=(0,))
+
+    large = None
+    if M8 is not None:
+        log("[bench] 8192x8192 reconstruction")
+        large = large_bench(L, M8)
+        del M8
+
+    fftc = None
+    if not args.no_fft_compare:
+        from imresh_amd import fft_compare
+        log("[bench] 2-D FFT vs rocFFT")
+        fftc = [fft_compare.compare(s_) for s_ in (2048, 8192)]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -379,7 +434,7 @@ def main() -> int:
                    "iterations_per_step": N_CYCLES * N_HIO, "width": W, "height": H,
                    "row_variant": rv.value, "col_variant": cv.value},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batch,
-        "serving": serve,
+        "serving": serve, "large_8192": large, "fft_vs_rocfft": fftc,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -142,6 +142,8 @@ IMR_DECLARE_GROUP(1)
 IMR_DECLARE_GROUP(2)
 IMR_DECLARE_GROUP(3)
 IMR_DECLARE_GROUP(4)
+IMR_DECLARE_GROUP(5)
+IMR_DECLARE_GROUP(6)
 // number of row workgroups of the plan's row variant
 int row_grid(const Plan2D &p);
 int blur_grid(int W, int H);

@@ -155,8 +155,9 @@ template <int N, int... Rs>
 constexpr int col_shape_count()
 {
     using P16 = PlanE<16, N, Rs...>;
-    if constexpr (N >= 1024 && P16::C > 4) return 5;
-    else if constexpr (N >= 1024) return 2;
+    constexpr bool pow2 = (N & (N - 1)) == 0;   // extra shapes for the power-of-two lengths
+    if constexpr (N >= 1024 && pow2 && P16::C > 4) return 5;
+    else if constexpr (N >= 1024 && pow2) return 2;
     return 1;
 }
 
@@ -179,7 +180,7 @@ int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
     constexpr int ns = col_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;
-    if constexpr (N >= 1024) {
+    if constexpr (ns > 1) {
         using P32 = PlanE<32, N, Rs...>;
         if (shape == 1) return launch_pf<P32, P32::C>(pf, mode, a, s);
         if constexpr (ns == 5) {

@@ -201,7 +201,7 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
 template <int N, int... Rs>
 constexpr int row_shape_count()
 {
-    if constexpr (N >= 1024) {
+    if constexpr (N >= 1024 && (N & (N - 1)) == 0) {   // extra shapes for the power-of-two lengths
         using P16 = PlanE<16, N, Rs...>;
         return (2 * P16::L * P16::T <= 1024 &&
                 sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 4 : 2;
@@ -232,7 +232,7 @@ int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
     constexpr int ns = row_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;
-    if constexpr (N >= 1024) {
+    if constexpr (row_shape_count<N, Rs...>() > 1) {
         using P32 = PlanE<32, N, Rs...>;
         if (shape == 1) return launch_pf<P32, P32::L>(pf, mode, a, s);
         if constexpr (ns == 4) {
@@ -251,7 +251,7 @@ int row_variant_rows(int var)
     if (var == kBlueVariant) return P16::L;
     constexpr int ns = row_shape_count<N, Rs...>();
     const int shape = var >= ns ? var - ns : var;
-    if constexpr (N >= 1024) {
+    if constexpr (row_shape_count<N, Rs...>() > 1) {
         using P32 = PlanE<32, N, Rs...>;
         if (shape == 1) return P32::L;
         if (shape == 2) return 2 * P16::L;

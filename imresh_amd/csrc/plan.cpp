@@ -68,36 +68,55 @@ void append_twiddles(const std::vector<int> &rad, std::vector<float2> &tw)
     }
 }
 
-// forward DFT of a power-of-two length in double (iterative radix 2), for
-// the Bluestein tables
-void dft_pow2(std::vector<std::complex<double>> &a)
+// forward DFT in double of a length whose prime factors are small (the
+// compiled convolution lengths), recursive mixed radix; for the Bluestein
+// tables
+void dft_any(std::vector<std::complex<double>> &a)
 {
-    const size_t m = a.size();
-    for (size_t i = 1, j = 0; i < m; ++i) {
-        size_t bit = m >> 1;
-        for (; j & bit; bit >>= 1) j ^= bit;
-        j ^= bit;
-        if (i < j) std::swap(a[i], a[j]);
-    }
-    for (size_t len = 2; len <= m; len <<= 1) {
-        for (size_t i = 0; i < m; i += len)
-            for (size_t k = 0; k < len / 2; ++k) {
-                const double ang = -2.0 * M_PI * (double) k / (double) len;
-                const std::complex<double> w(std::cos(ang), std::sin(ang));
-                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
-                a[i + k] = u + v;
-                a[i + k + len / 2] = u - v;
+    const int m = (int) a.size();
+    if (m <= 1) return;
+    int p = 2;
+    while (p * p <= m && m % p != 0) ++p;
+    if (m % p != 0) p = m;   // prime length: direct sum
+    const int q = m / p;
+    if (q == 1) {
+        std::vector<std::complex<double>> out(m);
+        for (int k = 0; k < m; ++k) {
+            std::complex<double> s = 0;
+            for (int j = 0; j < m; ++j) {
+                const double ang = -2.0 * M_PI * (double) ((long long) j * k % m) / (double) m;
+                s += a[j] * std::complex<double>(std::cos(ang), std::sin(ang));
             }
+            out[k] = s;
+        }
+        a.swap(out);
+        return;
     }
+    std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(q));
+    for (int r = 0; r < p; ++r)
+        for (int j = 0; j < q; ++j) sub[r][j] = a[(size_t) j * p + r];
+    for (auto &v : sub) dft_any(v);
+    for (int k = 0; k < q; ++k)
+        for (int s2 = 0; s2 < p; ++s2) {
+            const int kk = k + q * s2;
+            std::complex<double> acc = 0;
+            for (int r = 0; r < p; ++r) {
+                const double ang = -2.0 * M_PI * (double) ((long long) r * kk % m) / (double) m;
+                acc += sub[r][k] * std::complex<double>(std::cos(ang), std::sin(ang));
+            }
+            a[kk] = acc;
+        }
 }
 
-// convolution length of a Bluestein line: smallest power of two >= 2n-1
-// with compiled kernels (at least 64)
+// convolution length of a Bluestein line: the smallest power of two
+// >= max(64, 2n-1).  (Smooth lengths 3*2^k / 5*2^k pad less but measured
+// slower per line on gfx950: their plans need thread counts that are not
+// multiples of the wave size.)
 int blue_m(int n)
 {
     int m = 64;
     while (m < 2 * n - 1) m <<= 1;
-    return m;
+    return find_info(m) != nullptr ? m : 0;
 }
 
 // Bluestein tables of length n (layout in blue_device.hpp), float from double
@@ -118,8 +137,8 @@ std::vector<float2> blue_tables(int n, int m)
             bi[m - j] = a[j];
         }
     }
-    dft_pow2(bf);
-    dft_pow2(bi);
+    dft_any(bf);
+    dft_any(bi);
     std::vector<float2> out;
     out.reserve((size_t) n + 2 * m);
     for (int j = 0; j < n; ++j) out.push_back(make_float2((float) a[j].real(), (float) a[j].imag()));
@@ -145,6 +164,7 @@ int get_line_plan(int device, int n, LinePlan *out)
         if (n < 1 || n > kMaxLine) return IMRESH_ERR_UNSUPPORTED_SIZE;
         // Bluestein: the convolution plan's kernels and twiddles + the tables
         const int m = blue_m(n);
+        if (m == 0) return IMRESH_ERR_UNSUPPORTED_SIZE;
         LinePlan lm;
         IMR_CHECK(get_line_plan(device, m, &lm));
         const std::vector<float2> tab = blue_tables(n, m);
@@ -234,6 +254,8 @@ int row_grid(const Plan2D &p)
     case 2: L = row_rows_g2(n, v); break;
     case 3: L = row_rows_g3(n, v); break;
     case 4: L = row_rows_g4(n, v); break;
+    case 5: L = row_rows_g5(n, v); break;
+    case 6: L = row_rows_g6(n, v); break;
     default: break;
     }
     return (p.H + L - 1) / L;
@@ -252,6 +274,8 @@ int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
     case 2: return launch_row_g2(mode, v, n, p.row, a, s);
     case 3: return launch_row_g3(mode, v, n, p.row, a, s);
     case 4: return launch_row_g4(mode, v, n, p.row, a, s);
+    case 5: return launch_row_g5(mode, v, n, p.row, a, s);
+    case 6: return launch_row_g6(mode, v, n, p.row, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
     }
 }
@@ -269,6 +293,8 @@ int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
     case 2: return launch_col_g2(mode, v, n, p.col, a, s);
     case 3: return launch_col_g3(mode, v, n, p.col, a, s);
     case 4: return launch_col_g4(mode, v, n, p.col, a, s);
+    case 5: return launch_col_g5(mode, v, n, p.col, a, s);
+    case 6: return launch_col_g6(mode, v, n, p.col, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
     }
 }
@@ -282,6 +308,8 @@ int row_variants(const LinePlan &lp)
     case 2: return row_variants_g2(lp.n);
     case 3: return row_variants_g3(lp.n);
     case 4: return row_variants_g4(lp.n);
+    case 5: return row_variants_g5(lp.n);
+    case 6: return row_variants_g6(lp.n);
     default: return 0;
     }
 }
@@ -295,6 +323,8 @@ int col_variants(const LinePlan &lp)
     case 2: return col_variants_g2(lp.n);
     case 3: return col_variants_g3(lp.n);
     case 4: return col_variants_g4(lp.n);
+    case 5: return col_variants_g5(lp.n);
+    case 6: return col_variants_g6(lp.n);
     default: return 0;
     }
 }

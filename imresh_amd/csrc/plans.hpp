@@ -14,9 +14,13 @@
     X(1, 128, 16, 8) X(1, 256, 16, 16) X(1, 512, 8, 8, 8)                       \
     X(2, 1024, 16, 8, 8) X(2, 2048, 16, 16, 8)                                  \
     X(3, 4096, 16, 16, 16) X(3, 8192, 16, 8, 8, 8)                              \
-    X(4, 96, 8, 4, 3) X(4, 200, 8, 5, 5) X(4, 384, 16, 8, 3) X(4, 768, 16, 16, 3)
+    X(4, 96, 8, 4, 3) X(4, 200, 8, 5, 5) X(4, 384, 16, 8, 3) X(4, 768, 16, 16, 3)     \
+    X(5, 1280, 16, 16, 5) X(5, 1536, 16, 8, 4, 3) X(5, 2560, 16, 16, 5, 2)             \
+    X(5, 3072, 16, 16, 4, 3)                                                           \
+    X(6, 160, 16, 5, 2) X(6, 192, 16, 4, 3) X(6, 320, 16, 5, 4) X(6, 640, 16, 8, 5)   \
+    X(6, 5120, 16, 16, 5, 4) X(6, 6144, 16, 16, 8, 3)
 
-#define IMR_NUM_PLAN_GROUPS 5
+#define IMR_NUM_PLAN_GROUPS 7
 
 namespace imr {
 

@@ -46,7 +46,8 @@ def _gpu_iterations(M, g, gp, mask, beta, n_iter):
     return dg.download(np.complex64, (h, w)), dgp.download(np.complex64, (h, w))
 
 
-@pytest.mark.parametrize("w,h", [(64, 64), (256, 256), (96, 200), (1024, 1024), (2048, 2048)])
+@pytest.mark.parametrize("w,h", [(64, 64), (256, 256), (96, 200), (1024, 1024), (2048, 2048),
+                                 (1280, 1536), (640, 3072)])
 def test_one_hio_iteration_matches_oracle(w, h):
     obj, M, g, gp, mask = _state(w, h, index=w + h)
     g_gpu, gp_gpu = _gpu_iterations(M, g, gp, mask, 0.9, 1)
@@ -211,3 +212,17 @@ def test_config_2048_full_run_deterministic():
     assert np.array_equal(outs[0], outs[1])
     assert np.all(np.isfinite(outs[0]))
     print(f"2048^2 20x20: final error {res.last_error:.6g}, NCC to object {syn.ncc(obj, outs[0]):.4f}")
+
+
+def test_one_hio_iteration_8192():
+    """The large-image configuration's shape (8192^2: 8192-point rows in one
+    512-thread workgroup, 2-column tiles of 8192-point columns)."""
+    oracle.set_fft_threads(16)
+    obj, M, g, gp, mask = _state(8192, 8192, index=8192, shapes=64)
+    g_gpu, gp_gpu = _gpu_iterations(M, g, gp, mask, 0.9, 1)
+    g64, gp64 = oracle.hio_iterations(g.astype(np.complex128), gp.astype(np.complex128),
+                                      mask.astype(np.float64), M.astype(np.float64), 0.9, 1,
+                                      np.float64)
+    e64 = syn.rel_l2(g_gpu, g64)
+    print(f"8192x8192: one iteration rel L2 vs fp64 oracle {e64:.3e}")
+    assert e64 < 1e-5

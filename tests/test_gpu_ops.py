@@ -278,7 +278,8 @@ def test_blur_too_wide_kernel_is_an_error():
 # ------------------------------------------------------------ FFT
 FFT_SIZES = [(1, 1), (2, 4), (8, 8), (3, 5), (12, 60), (16, 32), (64, 64), (128, 256),
              (96, 200), (384, 768), (512, 512), (1024, 64), (2048, 16), (16, 2048),
-             (4096, 8), (8, 8192)]
+             (4096, 8), (8, 8192), (160, 192), (320, 640), (1280, 1536), (2560, 8), (8, 3072),
+             (5120, 4), (4, 6144)]
 
 
 def _fft_gpu(x, direction):
