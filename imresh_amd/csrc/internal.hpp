@@ -107,8 +107,14 @@ struct LinePlan {
     const float2 *bs;   // Bluestein tables (device)
 };
 
-// variant index of the Bluestein kernels (one shape per convolution length)
+// variant index of the Bluestein kernels: kBlueVariant + {0: the plan's
+// column tile, 1: half as many columns per workgroup, 2 / 3: the same with
+// the prefetching junction}; rows use kBlueVariant (+1 prefetching).  Sizes
+// without a plan are not tuned one by one (a server meets hundreds of them):
+// half-width column tiles measured fastest on every Bluestein size tried
+// (300-2039: 1-1.4x), so they are the fixed choice.
 constexpr int kBlueVariant = 1000;
+constexpr int kBlueColDefault = 1;
 constexpr int kMaxLine = 4096;   // longest line a Bluestein plan serves
 
 struct Plan2D {

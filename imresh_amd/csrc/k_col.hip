@@ -175,8 +175,18 @@ template <int N, int... Rs>
 int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
 {
     using P16 = PlanE<16, N, Rs...>;
-    if constexpr (blue_length<N>())
-        if (var == kBlueVariant) return launch_variant<P16, P16::C, false, false, true>(mode, a, s);
+    if constexpr (blue_length<N>()) {
+        // Bluestein variants: the plan's tile or half as many columns (twice
+        // the workgroups, half the LDS), each plain or prefetching
+        constexpr int C2 = P16::C >= 2 ? P16::C / 2 : 1;
+        switch (var - kBlueVariant) {
+        case 0: return launch_variant<P16, P16::C, false, false, true>(mode, a, s);
+        case 1: return launch_variant<P16, C2, false, false, true>(mode, a, s);
+        case 2: return launch_variant<P16, P16::C, false, true, true>(mode, a, s);
+        case 3: return launch_variant<P16, C2, false, true, true>(mode, a, s);
+        default: break;
+        }
+    }
     constexpr int ns = col_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;

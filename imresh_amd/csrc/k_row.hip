@@ -227,8 +227,10 @@ template <int N, int... Rs>
 int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
 {
     using P16 = PlanE<16, N, Rs...>;
-    if constexpr (blue_length<N>())
+    if constexpr (blue_length<N>()) {
         if (var == kBlueVariant) return launch_variant<P16, P16::L, false, true>(mode, a, s);
+        if (var == kBlueVariant + 1) return launch_variant<P16, P16::L, true, true>(mode, a, s);
+    }
     constexpr int ns = row_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;
@@ -248,7 +250,7 @@ template <int N, int... Rs>
 int row_variant_rows(int var)
 {
     using P16 = PlanE<16, N, Rs...>;
-    if (var == kBlueVariant) return P16::L;
+    if (var >= kBlueVariant) return P16::L;
     constexpr int ns = row_shape_count<N, Rs...>();
     const int shape = var >= ns ? var - ns : var;
     if constexpr (row_shape_count<N, Rs...>() > 1) {

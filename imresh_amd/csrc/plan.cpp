@@ -286,7 +286,7 @@ int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
     a.len = p.H;
     a.bs = p.col.bs;
     const int n = p.col.blue ? p.col.m : p.H;
-    const int v = p.col.blue ? kBlueVariant : p.colVar;
+    const int v = p.col.blue ? kBlueVariant + kBlueColDefault : p.colVar;
     switch (p.col.group) {
     case 0: return launch_col_g0(mode, v, n, p.col, a, s);
     case 1: return launch_col_g1(mode, v, n, p.col, a, s);
@@ -301,7 +301,7 @@ int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
 
 int row_variants(const LinePlan &lp)
 {
-    if (lp.blue) return 1;
+    if (lp.blue) return 1;   // Bluestein: fixed choice (no per-size tuning, see kBlueColDefault)
     switch (lp.group) {
     case 0: return row_variants_g0(lp.n);
     case 1: return row_variants_g1(lp.n);
