@@ -100,6 +100,10 @@ _SIGS = {
     "imresh_server_drain": (c_int, [_P]),
     "imresh_server_stats": (c_int, [_P, POINTER(ServerInfo)]),
     "imresh_server_destroy": (c_int, [_P]),
+    "imresh_hybrid_input_output": (c_int, [_P, _P, c_uint32, c_uint32, c_uint32, c_float, c_float,
+                                           _P, POINTER(Result)]),
+    "imresh_hybrid_input_output_device": (c_int, [_P, _P, c_uint32, c_uint32, c_uint32, c_float,
+                                                  c_float, _P, POINTER(Result)]),
     "imresh_hio_iterations": (c_int, [_P, _P, _P, _P, c_uint32, c_uint32, c_float, c_uint32, _P]),
     "imresh_copy_to_real_part": (c_int, [_P, _P, c_uint32, _P]),
     "imresh_copy_from_real_part": (c_int, [_P, _P, c_uint32, _P]),
@@ -294,6 +298,24 @@ def shrink_wrap_batch_device(images: list, params: Params | None = None, stream=
                                                 res if want_results else None),
           "imresh_shrinkwrap_batch_device")
     return [res[i] for i in range(len(images))] if want_results else None
+
+
+def hybrid_input_output(io: np.ndarray, mask: np.ndarray, n_cycles: int = 0,
+                        target_error: float = 1e-6, hio_beta: float = 0.9, stream=None) -> Result:
+    """imresh::libs::hybridInputOutput on the GPU: `io` (float32 H x W) holds
+    |F| and is overwritten with Re(g'); `mask` uint8 H x W, 1 = outside the
+    support; n_cycles 0 = until target_error is met."""
+    if io.dtype != np.float32 or not io.flags.c_contiguous or io.ndim != 2:
+        raise ValueError("io must be a C-contiguous float32 H x W array")
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    if m.shape != io.shape:
+        raise ValueError("mask must have the shape of io")
+    h, w = io.shape
+    r = Result()
+    check(_lib().imresh_hybrid_input_output(io.ctypes.data, m.ctypes.data, w, h, n_cycles,
+                                            target_error, hio_beta, stream, ctypes.byref(r)),
+          "imresh_hybrid_input_output")
+    return r
 
 
 class Server:

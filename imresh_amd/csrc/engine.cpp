@@ -19,6 +19,7 @@
 // The early exit sets a device flag that turns every later kernel of the
 // reconstruction into a no-op, so the host never waits inside the loop.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -804,6 +805,152 @@ int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
     }
     ws_return(w, s);
     return st;
+}
+
+}  // extern "C"
+
+namespace {
+
+// libs::hybridInputOutput (hybridInputOutput.cpp:200-306) on a workspace whose
+// M holds |F| and mask the caller's support (uint8, 1 = outside):
+//   random phase (:126-197) -> inverse 2-D FFT, gPrev = g' -> per iteration
+//   [constraint -> FFT -> modulus -> inverse FFT], with the error check after
+//   every inverse transform when target > 0 (fused row passes otherwise).
+// Leaves g' in X; cycles_run = HIO iterations executed.
+int run_hio_standalone(Workspace *w, unsigned n_cycles, float target, float beta, hipStream_t s,
+                       imresh_result *res)
+{
+    const size_t n = w->n;
+    Plan2D p;
+    IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
+    // the random real field: glibc rand() after srand(2623091912), as :179-184
+    {
+        std::vector<float> field(n);
+        srand(2623091912u);
+        for (size_t i = 0; i < n; ++i) field[i] = (float) rand() / RAND_MAX;
+        IMR_HIP_CHECK(hipMemcpyAsync(w->B2, field.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+        IMR_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    IMR_CHECK(launch_copy_to_real(w->X, w->B2, n, s));
+    RowArgs rf{};
+    rf.in = w->X;
+    rf.out = w->X;
+    IMR_CHECK(launch_row(ROW_FWD, p, rf, s));
+    ColArgs cf{};
+    cf.in = w->X;
+    cf.out = w->X;
+    IMR_CHECK(launch_col(COL_FWD, p, cf, s));
+    IMR_CHECK(launch_random_phase(w->X, w->M, n, s));
+    IMR_HIP_CHECK(hipMemsetAsync(w->flags, 0, 64, s));
+    IMR_HIP_CHECK(hipMemsetAsync(w->scal, 0, 64, s));
+    // first inverse transform; gPrevious = g' (:238-239)
+    cf.in = w->X;
+    IMR_CHECK(launch_col(COL_INV, p, cf, s));
+    Params q;
+    normalize(nullptr, q);
+    q.beta = beta;
+    RowArgs ra = row_args(w, q);
+    const ColArgs ca = col_args(w, w->M);
+    const bool check = target > 0;
+    const int nrow = row_grid(p);
+    auto finalize = [&] {
+        return launch_err_finalize(w->perr, w->pcnt, nrow, target, w->scal + 1, w->flags,
+                                   (unsigned *) (w->flags + 1), s);
+    };
+    IMR_CHECK(launch_row(check ? ROW_INIT_ERR : ROW_INIT, p, ra, s));
+    if (check) IMR_CHECK(finalize());
+    constexpr unsigned kChunk = 64;   // iterations queued between done-flag reads
+    unsigned done_iters = 0;
+    for (unsigned c = 0; c < n_cycles;) {
+        const unsigned chunk = std::min(kChunk, n_cycles - c);
+        for (unsigned k = 0; k < chunk; ++k, ++c) {
+            if (check) {
+                IMR_CHECK(launch_row(ROW_FIRST_MASK, p, ra, s));
+                IMR_CHECK(launch_col(COL_PROJ, p, ca, s));
+                IMR_CHECK(launch_row(ROW_LAST, p, ra, s));
+                IMR_CHECK(finalize());
+            } else {
+                IMR_CHECK(launch_row(c == 0 ? ROW_FIRST_MASK : ROW_MID, p, ra, s));
+                IMR_CHECK(launch_col(COL_PROJ, p, ca, s));
+            }
+        }
+        done_iters = c;
+        if (check && c < n_cycles) {   // stop queueing once the target is met
+            IMR_CHECK(recon_stage_results(w, s));
+            IMR_HIP_CHECK(hipStreamSynchronize(s));
+            int d = 0;
+            recon_read_staged(w, nullptr, &d);
+            if (d) break;
+        }
+    }
+    if (!check && done_iters > 0) IMR_CHECK(launch_row(ROW_LAST_PLAIN, p, ra, s));
+    IMR_CHECK(recon_stage_results(w, s));
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    imresh_result r{};
+    recon_read_staged(w, &r, nullptr);
+    if (check) {
+        r.cycles_run = r.cycles_run > 0 ? r.cycles_run - 1 : 0;
+    } else {
+        r.cycles_run = done_iters;
+        r.last_error = r.last_total = r.last_masked = 0.f;
+    }
+    if (res) *res = r;
+    return IMRESH_OK;
+}
+
+int hio_standalone_entry(float *io, const uint8_t *mask, uint32_t W, uint32_t H, uint32_t n_cycles,
+                         float target, float beta, void *stream, imresh_result *result, bool device_io)
+{
+    if (io == nullptr || mask == nullptr) return IMRESH_ERR_NULL;
+    if (W == 0 || H == 0) return IMRESH_ERR_INVALID;
+    if (n_cycles == 0) n_cycles = UINT_MAX;   // :218
+    if (beta <= 0) beta = 0.9f;               // :219
+    if (n_cycles == UINT_MAX && !(target > 0)) return IMRESH_ERR_INVALID;   // would never end
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, W, H, &p));
+    hipStream_t s = S(stream);
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, W, H, s, &w));
+    const size_t n = w->n;
+    const hipMemcpyKind in_kind = device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    int st = ws_tune(w, s);
+    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(w->M, io, sizeof(float) * n, in_kind, s));
+    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(w->mask, mask, n, in_kind, s));
+    if (st == IMRESH_OK) st = run_hio_standalone(w, n_cycles, target, beta, s, result);
+    if (st == IMRESH_OK) {
+        if (device_io) {
+            st = launch_copy_from_real(io, w->X, n, s);
+        } else {
+            st = launch_copy_from_real(w->B2, w->X, n, s);
+            if (st == IMRESH_OK)
+                st = from_hip(hipMemcpyAsync(io, w->B2, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        }
+    }
+    if (st == IMRESH_OK) st = from_hip(hipStreamSynchronize(s));
+    ws_return(w, s);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int imresh_hybrid_input_output(float *io, const uint8_t *mask, uint32_t width, uint32_t height,
+                               uint32_t n_cycles, float target_error, float hio_beta, void *stream,
+                               imresh_result *result)
+{
+    return hio_standalone_entry(io, mask, width, height, n_cycles, target_error, hio_beta, stream,
+                                result, false);
+}
+
+int imresh_hybrid_input_output_device(float *io_dev, const uint8_t *mask_dev, uint32_t width,
+                                      uint32_t height, uint32_t n_cycles, float target_error,
+                                      float hio_beta, void *stream, imresh_result *result)
+{
+    return hio_standalone_entry(io_dev, mask_dev, width, height, n_cycles, target_error, hio_beta,
+                                stream, result, true);
 }
 
 int imresh_hio_iterations(float *gprime_dev, float *gprev_dev, const uint8_t *mask_dev,

@@ -20,6 +20,7 @@ enum RowMode {
     ROW_MID,            // X -> inverse -> constraint(mask, G) -> forward -> X
     ROW_LAST,           // X -> inverse -> X = g', Bn = |g'|, error partials
     ROW_LAST_PLAIN,     // X -> inverse -> X = g'
+    ROW_INIT_ERR,       // ROW_INIT + error partials
     ROW_NMODES
 };
 
@@ -166,6 +167,7 @@ int launch_max_finalize(const float *pmax, int np, float cut, float *thr, const 
 int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float target,
                         float *res, int *done, unsigned *cycles, hipStream_t s);
 int launch_copy_from_real(float *out, const float2 *in, size_t n, hipStream_t s);
+int launch_random_phase(float2 *X, const float *M, size_t n, hipStream_t s);
 int launch_gather_images(float *dst, const float *const *src, size_t n, int batch, hipStream_t s);
 int launch_scatter_real(float *const *dst, const float2 *X, size_t n, int batch, hipStream_t s);
 int launch_copy_to_real(float2 *out, const float *in, size_t n, hipStream_t s);

@@ -495,6 +495,27 @@ int launch_err_finalize_batch(const float *perr, const unsigned *pcnt, int np, f
     return from_hip(hipGetLastError());
 }
 
+// addRandomPhase (hybridInputOutput.cpp:186-195): X holds the forward FFT of
+// the random real field; X = |F| * (cos, sin)(arg X), the phase a float
+// atan2 and cos/sin evaluated in double as the reference's unqualified cos
+// of a float
+__global__ void __launch_bounds__(256) k_random_phase(float2 *X, const float *M, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n;
+         i += (size_t) gridDim.x * blockDim.x) {
+        const float2 z = X[i];
+        const float phase = atan2f(z.y, z.x);
+        const double m = (double) M[i];
+        X[i] = make_float2((float) (m * cos((double) phase)), (float) (m * sin((double) phase)));
+    }
+}
+
+int launch_random_phase(float2 *X, const float *M, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_random_phase, dim3(grid_1d(n)), dim3(256), 0, s, X, M, n);
+    return from_hip(hipGetLastError());
+}
+
 // batch staging: image b of a pointer table <-> plane b of a batch buffer
 __global__ void __launch_bounds__(256) k_gather_images(float *dst, const float *const *src, size_t n)
 {

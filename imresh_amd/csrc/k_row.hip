@@ -180,6 +180,10 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         a.flags = 0;
         IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
         break;
+    case ROW_INIT_ERR:
+        a.flags = kRowWriteG | kRowWriteBn | kRowError;
+        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
+        break;
     case ROW_FIRST_CUT:
         a.flags = kRowCut;
         IMR_LAUNCH((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "imresh/algorithms/shrinkWrap.hpp"
+#include "imresh/libs/hybridInputOutput.hpp"
 #include "imresh/io/taskQueue.hpp"
 #include "internal.hpp"
 
@@ -214,4 +215,18 @@ void addTask(float *const ioData, unsigned int const imageWidth, unsigned int co
 }
 
 }  // namespace io
+namespace libs {
+
+int hybridInputOutput(float *const rIoData, uint8_t const *const rIsMasked,
+                      std::vector<unsigned int> const rSize, unsigned int rnCycles, float rTargetError,
+                      float rBeta, unsigned int /* rnCores */)
+{
+    if (rSize.size() != 2) return 1;   // hybridInputOutput.cpp:209
+    if (rIoData == nullptr || rIsMasked == nullptr) return 1;
+    const int st = imresh_hybrid_input_output(rIoData, rIsMasked, rSize[0], rSize[1], rnCycles,
+                                              rTargetError, rBeta, nullptr, nullptr);
+    return st == IMRESH_OK ? 0 : st;
+}
+
+}  // namespace libs
 }  // namespace imresh

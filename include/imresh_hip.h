@@ -184,6 +184,24 @@ int imresh_server_stats(imresh_server *server, imresh_server_info *info);
 /* Drains, stops the scheduler and frees the device state. */
 int imresh_server_destroy(imresh_server *server);
 
+/* Standalone HIO with a given support (libs::hybridInputOutput,
+ * src/imresh/libs/hybridInputOutput.cpp:200-306): io = |F| in, Re(g') out;
+ * mask uint8 (1 = outside the support, :267); the first guess is |F| with
+ * the phases of the FFT of glibc rand()/RAND_MAX after srand(2623091912)
+ * (:126-197, like the reference this reseeds the process's rand());
+ * n_cycles 0 -> unbounded (UINT_MAX), hio_beta <= 0 -> 0.9; with
+ * target_error > 0 the HIO error is checked after every inverse transform
+ * and the loop stops below it (n_cycles 0 with no target: IMRESH_ERR_INVALID
+ * instead of the reference's endless loop).  result->cycles_run = HIO
+ * iterations run.  The modulus projection maps |G'| = 0 to 0 (the reference
+ * divides by zero there).  Synchronous. */
+int imresh_hybrid_input_output(float *io, const uint8_t *mask, uint32_t width, uint32_t height,
+                               uint32_t n_cycles, float target_error, float hio_beta, void *stream,
+                               imresh_result *result);
+int imresh_hybrid_input_output_device(float *io_dev, const uint8_t *mask_dev, uint32_t width,
+                                      uint32_t height, uint32_t n_cycles, float target_error,
+                                      float hio_beta, void *stream, imresh_result *result);
+
 /* HIO inner loop from a given device state, n_iter iterations
  * (cudaShrinkWrap.cu:311-365 / shrinkWrap.cpp:260-319): gprime (complex, g'),
  * gprev (complex), mask (uint8, 1 = outside support), modulus (float).

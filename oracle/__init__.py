@@ -56,6 +56,9 @@ def _declare(lib):
             "orc_hio_iterations": (c_int, [P, P, P, P, c_uint, c_uint, ct, c_uint]),
             "orc_shrinkwrap": (c_int, [P, c_uint, c_uint, c_uint, c_float, c_float, c_float,
                                        c_float, c_float, c_float, c_uint, POINTER(SwStats)]),
+            "orc_random_phase": (None, [P, P, c_uint, c_uint]),
+            "orc_hio_standalone": (c_int, [P, P, c_uint, c_uint, c_uint, c_float, c_float,
+                                           POINTER(SwStats)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, f"{name}_{s}")
@@ -255,6 +258,29 @@ def shrink_wrap(modulus, n_cycles=20, target_error=1e-5, hio_beta=0.9, cutoff_au
         sigma_change, n_hio_cycles, ctypes.byref(st))
     if rc != 0:
         raise RuntimeError(f"orc_shrinkwrap returned {rc}")
+    return io, st
+
+
+def random_phase(norm, dtype=np.float32):
+    """addRandomPhase (hybridInputOutput.cpp:126-197): |F| * exp(i arg FFT(rand field))."""
+    M = _real(norm, dtype)
+    h, w = M.shape
+    out = np.zeros(2 * w * h, dtype=dtype)
+    getattr(lib(), f"orc_random_phase_{_sfx(dtype)}")(_p(M), _p(out), w, h)
+    return _to_complex(out, dtype, (h, w))
+
+
+def hio_standalone(modulus, mask, n_cycles=0, target_error=1e-6, beta=0.9, dtype=np.float32):
+    """libs::hybridInputOutput (hybridInputOutput.cpp:200-306) with a given
+    uint8 mask.  Returns (object, SwStats) with cycles_run = HIO iterations."""
+    io = np.array(modulus, dtype=dtype, copy=True)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    h, w = io.shape
+    st = SwStats()
+    rc = getattr(lib(), f"orc_hio_standalone_{_sfx(dtype)}")(_p(io), _p(m), w, h, n_cycles,
+                                                               target_error, beta, ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_hio_standalone returned {rc}")
     return io, st
 
 

@@ -14,6 +14,7 @@
  *   gaussian blur         src/imresh/libs/gaussian.cpp:51-197,559-749
  *   calcGaussianKernel    src/imresh/libs/calcGaussianKernel.cpp:42-91
  *   calculateHioError     src/imresh/libs/hybridInputOutput.cpp:67-103
+ *   hybridInputOutput     src/imresh/libs/hybridInputOutput.cpp:126-306
  *   CUDA kernels          src/imresh/algorithms/cuda/cudaVectorElementwise.cu:42-219
  * The FFT (FFTW3f in the reference, absent here) is replaced by a
  * project-written mixed-radix + Bluestein CPU FFT, unnormalized, with
@@ -67,7 +68,11 @@ typedef struct orc_sw_stats {
                             float target_error, float hio_beta,                    \
                             float cutoff_autocorel, float cutoff, float sigma0,    \
                             float sigma_change, unsigned n_hio,                    \
-                            orc_sw_stats *stats);
+                            orc_sw_stats *stats);                                  \
+    void orc_random_phase_##S(const T *norm, T *out, unsigned w, unsigned h);      \
+    int  orc_hio_standalone_##S(T *io, const unsigned char *mask, unsigned w,      \
+                                unsigned h, unsigned n_cycles, float target,       \
+                                float beta, orc_sw_stats *stats);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

@@ -471,3 +471,81 @@ int FN(orc_shrinkwrap)(T *io, unsigned w, unsigned h, unsigned n_cycles,
     free(cur); free(gprev); free(mask);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* standalone HIO with a given support (hybridInputOutput.cpp:126-306)       */
+/* ------------------------------------------------------------------------ */
+/* addRandomPhase (:126-197): srand(2623091912); a real field rand()/RAND_MAX,
+ * its forward FFT, phase = arg (float atan2), output = |F| * (cos, sin) of
+ * the phase evaluated in double (unqualified cos on a float) */
+void FN(orc_random_phase)(const T *norm, T *out, unsigned w, unsigned h)
+{
+    const unsigned n = w * h;
+    T *tmp = (T *) malloc(sizeof(T) * 2 * (size_t) n);
+    srand(2623091912u);
+    for (unsigned i = 0; i < n; ++i) {
+        tmp[2 * i] = (T) ((float) rand() / RAND_MAX);
+        tmp[2 * i + 1] = 0;
+    }
+    FN(orc_fft2d)(tmp, tmp, w, h, -1);
+    for (unsigned i = 0; i < n; ++i) {
+        const float phase = atan2f((float) tmp[2 * i + 1], (float) tmp[2 * i]);
+        out[2 * i] = (T) ((double) norm[i] * cos((double) phase));
+        out[2 * i + 1] = (T) ((double) norm[i] * sin((double) phase));
+    }
+    free(tmp);
+}
+
+/* hybridInputOutput (:200-306): io = |F| in, Re(g') out; mask uint8 (1 =
+ * outside the support); n_cycles 0 -> UINT_MAX, beta <= 0 -> 0.9; the error
+ * check runs after every inverse transform when target > 0.  The modulus
+ * replacement divides by |G'| without the zero guard of applyComplexModulus
+ * (:292-297). */
+int FN(orc_hio_standalone)(T *io, const unsigned char *mask8, unsigned w, unsigned h,
+                           unsigned n_cycles, float target, float beta, orc_sw_stats *stats)
+{
+    if (io == NULL || mask8 == NULL) return 1;
+    if (n_cycles == 0) n_cycles = UINT_MAX;
+    if (beta <= 0) beta = 0.9f;
+    const unsigned n = w * h;
+    T *cur = (T *) malloc(sizeof(T) * 2 * (size_t) n);
+    T *gprev = (T *) malloc(sizeof(T) * 2 * (size_t) n);
+    T *mask = (T *) malloc(sizeof(T) * (size_t) n);
+    for (unsigned i = 0; i < n; ++i) mask[i] = (T) (mask8[i] ? 1 : 0);
+    FN(orc_random_phase)(io, cur, w, h);
+    unsigned it = 0;
+    float err = 0, tot = 0, nm = 0;
+    for (;;) {
+        FN(orc_fft2d)(cur, cur, w, h, +1);
+        if (it == 0) memcpy(gprev, cur, sizeof(T) * 2 * (size_t) n);
+        if (target > 0) {
+            err = FN(orc_hio_error)(cur, mask, n, 0, &tot, &nm);
+            if (err < target) break;
+        }
+        if (it >= n_cycles) break;
+        for (unsigned i = 0; i < n; ++i) {
+            if (mask8[i] == 1 || cur[2 * i] < 0) {
+                cur[2 * i] = gprev[2 * i] - (T) beta * cur[2 * i];
+                cur[2 * i + 1] = gprev[2 * i + 1] - (T) beta * cur[2 * i + 1];
+            }
+        }
+        memcpy(gprev, cur, sizeof(T) * 2 * (size_t) n);
+        FN(orc_fft2d)(cur, cur, w, h, -1);
+        for (unsigned i = 0; i < n; ++i) {
+            const T re = cur[2 * i], im = cur[2 * i + 1];
+            const T factor = io[i] / (T) SQRT(re * re + im * im);
+            cur[2 * i] *= factor;
+            cur[2 * i + 1] *= factor;
+        }
+        ++it;
+    }
+    FN(orc_copy_from_real)(io, cur, n);
+    if (stats) {
+        stats->cycles_run = it;
+        stats->last_error = err;
+        stats->last_total = tot;
+        stats->last_masked = nm;
+    }
+    free(cur); free(gprev); free(mask);
+    return 0;
+}

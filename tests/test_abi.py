@@ -43,7 +43,8 @@ def test_exports_cpp_dropin_api():
     for sym in ("imresh::algorithms::shrinkWrap(float*, unsigned int, unsigned int, unsigned int, "
                 "float, float, float, float, float, float, unsigned int)",
                 "imresh::io::taskQueueInit()", "imresh::io::taskQueueDeinit()",
-                "imresh::io::addTask("):
+                "imresh::io::addTask(",
+                "imresh::libs::hybridInputOutput(float*, unsigned char const*, std::vector<unsigned int"):
         assert sym in out, sym
 
 

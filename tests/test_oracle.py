@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import oracle
+from imresh_amd import synthetic as syn
 from helpers import FLT_EPS, KAT_SEED, RAND_MAX, compare_float_array, log_spaced_points, rand, srand
 
 REF = oracle.ref()
@@ -351,3 +352,58 @@ def test_golden_fixtures():
     out, st = oracle.shrink_wrap(z["sw_modulus"], n_cycles=2, n_hio_cycles=3,
                                  target_error=1e-38, dtype=np.float64)
     assert np.allclose(out, z["sw_out_f64"], rtol=1e-9, atol=1e-9)
+
+
+# ---------------------------------------------------------------- standalone HIO
+def test_random_phase_matches_glibc_field_and_numpy_fft():
+    """addRandomPhase (hybridInputOutput.cpp:126-197): the real field is
+    (float) rand() / RAND_MAX after srand(2623091912) (glibc's generator,
+    drawn here through ctypes), its
+    phases come from a forward FFT (numpy, float64 here) and the output keeps
+    the input modulus."""
+    import helpers
+    h, w = 24, 40
+    rng = np.random.default_rng(5)
+    M = rng.uniform(0.5, 2.0, (h, w)).astype(np.float32)
+    helpers.srand(2623091912)
+    field = np.array([np.float32(helpers.rand()) / np.float32(helpers.RAND_MAX) for _ in range(h * w)],
+                     dtype=np.float32).reshape(h, w)
+    F = np.fft.fft2(field.astype(np.float64))
+    expect = M.astype(np.float64) * np.exp(1j * np.angle(F))
+    got = oracle.random_phase(M, np.float64)
+    assert np.allclose(np.abs(got), M, rtol=1e-12)
+    assert np.max(np.abs(got - expect)) < 1e-5
+    got32 = oracle.random_phase(M, np.float32)
+    assert np.max(np.abs(got32 - expect)) < 1e-4
+
+
+def test_hio_standalone_is_random_phase_then_hio_iterations():
+    """hybridInputOutput (:200-306) = random phase -> inverse FFT -> gPrev =
+    g' -> n HIO iterations; with target <= 0 it must equal that composition of
+    the pinned operators exactly (float64, no exact zeros in |G'|)."""
+    obj, M = syn.workload(32, 48, n_shapes=4, central=16, index=4)
+    mask = (obj == 0).astype(np.uint8)
+    n = 4
+    out, st = oracle.hio_standalone(M, mask, n_cycles=n, target_error=0, dtype=np.float64)
+    assert st.cycles_run == n
+    g = oracle.fft2d(oracle.random_phase(M, np.float64), +1, np.float64)
+    # the reference's beta is a float (0.9f), widened to the working precision
+    g, gp = oracle.hio_iterations(g, g.copy(), mask.astype(np.float64), M.astype(np.float64),
+                                  float(np.float32(0.9)), n, np.float64)
+    assert np.array_equal(out, g.real)
+
+
+def test_hio_standalone_early_exit_and_defaults():
+    obj, M = syn.workload(32, 32, n_shapes=4, central=16, index=9)
+    mask = (obj == 0).astype(np.uint8)
+    # a target met at the first check: no iteration, output = Re(IFFT(random phase))
+    out, st = oracle.hio_standalone(M, mask, n_cycles=0, target_error=1e30, dtype=np.float64)
+    assert st.cycles_run == 0
+    g = oracle.fft2d(oracle.random_phase(M, np.float64), +1, np.float64)
+    assert np.array_equal(out, g.real)
+    e, tot, nm = oracle.hio_error(g, mask.astype(np.float64), 0, np.float64)
+    assert st.last_error == np.float32(e) and st.last_masked == mask.sum()
+    # beta <= 0 selects 0.9
+    a, _ = oracle.hio_standalone(M, mask, n_cycles=2, target_error=0, beta=0, dtype=np.float64)
+    b, _ = oracle.hio_standalone(M, mask, n_cycles=2, target_error=0, beta=0.9, dtype=np.float64)
+    assert np.array_equal(a, b)
