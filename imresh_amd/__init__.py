@@ -124,6 +124,7 @@ _SIGS = {
     "imresh_gaussian_blur_vertical": (c_int, [_P, c_uint32, c_uint32, c_double, _P]),
     "imresh_fft2d": (c_int, [_P, _P, c_uint32, c_uint32, c_int, _P]),
     "imresh_fft_supported": (c_int, [c_uint32]),
+    "imresh_fft_shift": (c_int, [_P, c_uint32, c_uint32, c_uint32, _P]),
     "imresh_autotune": (c_int, [c_uint32, c_uint32, _P, POINTER(c_int), POINTER(c_int)]),
     "imresh_task_queue_init": (c_int, [c_int]),
     "imresh_task_queue_deinit": (c_int, []),

@@ -1090,6 +1090,12 @@ int imresh_fft2d(float *out, const float *in, uint32_t W, uint32_t H, int direct
 
 int imresh_fft_supported(uint32_t n) { return line_supported((int) n) ? 1 : 0; }
 
+int imresh_fft_shift(void *data_dev, uint32_t nx, uint32_t ny, uint32_t element_bytes, void *stream)
+{
+    if (data_dev == nullptr) return IMRESH_ERR_NULL;
+    return launch_fft_shift(data_dev, nx, ny, element_bytes, S(stream));
+}
+
 int imresh_autotune(uint32_t width, uint32_t height, void *stream, int *row_variant,
                     int *col_variant)
 {

@@ -168,6 +168,7 @@ int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float t
                         float *res, int *done, unsigned *cycles, hipStream_t s);
 int launch_copy_from_real(float *out, const float2 *in, size_t n, hipStream_t s);
 int launch_random_phase(float2 *X, const float *M, size_t n, hipStream_t s);
+int launch_fft_shift(void *d, unsigned nx, unsigned ny, unsigned elem_bytes, hipStream_t s);
 int launch_gather_images(float *dst, const float *const *src, size_t n, int batch, hipStream_t s);
 int launch_scatter_real(float *const *dst, const float2 *X, size_t n, int batch, hipStream_t s);
 int launch_copy_to_real(float2 *out, const float *in, size_t n, hipStream_t s);

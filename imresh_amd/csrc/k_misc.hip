@@ -516,6 +516,35 @@ int launch_random_phase(float2 *X, const float *M, size_t n, hipStream_t s)
     return from_hip(hipGetLastError());
 }
 
+// fftShift (fftShift.cpp:42-61): element (iy, ix), iy < ny/2, swaps with
+// ((iy + ny/2) % ny, (ix + nx/2) % nx); the pairs are disjoint, one thread each
+template <class E>
+__global__ void __launch_bounds__(256) k_fft_shift(E *d, unsigned nx, unsigned ny)
+{
+    const size_t half = (size_t) (ny / 2) * nx;
+    for (size_t k = blockIdx.x * (size_t) blockDim.x + threadIdx.x; k < half;
+         k += (size_t) gridDim.x * blockDim.x) {
+        const unsigned iy = (unsigned) (k / nx), ix = (unsigned) (k - (size_t) iy * nx);
+        const size_t b = (size_t) ((iy + ny / 2) % ny) * nx + (ix + nx / 2) % nx;
+        const E t = d[k];
+        d[k] = d[b];
+        d[b] = t;
+    }
+}
+
+int launch_fft_shift(void *d, unsigned nx, unsigned ny, unsigned elem_bytes, hipStream_t s)
+{
+    const size_t half = (size_t) (ny / 2) * nx;
+    if (half == 0) return IMRESH_OK;
+    if (elem_bytes == 4)
+        hipLaunchKernelGGL(k_fft_shift<float>, dim3(grid_1d(half)), dim3(256), 0, s, (float *) d, nx, ny);
+    else if (elem_bytes == 8)
+        hipLaunchKernelGGL(k_fft_shift<float2>, dim3(grid_1d(half)), dim3(256), 0, s, (float2 *) d, nx, ny);
+    else
+        return IMRESH_ERR_INVALID;
+    return from_hip(hipGetLastError());
+}
+
 // batch staging: image b of a pointer table <-> plane b of a batch buffer
 __global__ void __launch_bounds__(256) k_gather_images(float *dst, const float *const *src, size_t n)
 {

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "imresh/algorithms/shrinkWrap.hpp"
+#include "imresh/libs/fftShift.hpp"
 #include "imresh/libs/hybridInputOutput.hpp"
 #include "imresh/io/taskQueue.hpp"
 #include "internal.hpp"
@@ -227,6 +228,24 @@ int hybridInputOutput(float *const rIoData, uint8_t const *const rIsMasked,
                                               rTargetError, rBeta, nullptr, nullptr);
     return st == IMRESH_OK ? 0 : st;
 }
+
+template <class T_COMPLEX>
+void fftShift(T_COMPLEX *const data, unsigned int const Nx, unsigned int const Ny)
+{
+    // the reference returns nothing and asserts on a null pointer
+    if (data == nullptr || Nx == 0 || Ny == 0) return;
+    const size_t bytes = sizeof(T_COMPLEX) * (size_t) Nx * Ny;
+    void *d = nullptr;
+    if (imresh_dev_alloc(&d, bytes) != IMRESH_OK) return;
+    if (imresh_memcpy_h2d(d, data, bytes, nullptr) == IMRESH_OK &&
+        imresh_fft_shift(d, Nx, Ny, (uint32_t) sizeof(T_COMPLEX), nullptr) == IMRESH_OK)
+        imresh_memcpy_d2h(data, d, bytes, nullptr);
+    imresh_stream_sync(nullptr);
+    imresh_dev_free(d);
+}
+
+template void fftShift<float>(float *const, unsigned int const, unsigned int const);
+template void fftShift<float[2]>(float (*const)[2], unsigned int const, unsigned int const);
 
 }  // namespace libs
 }  // namespace imresh

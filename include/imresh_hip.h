@@ -260,6 +260,10 @@ int imresh_fft2d(float *out_dev, const float *in_dev, uint32_t width, uint32_t h
                  int direction, void *stream);
 /* 1 if an FFT plan exists for this line length, else 0. */
 int imresh_fft_supported(uint32_t n);
+/* fftShift (src/imresh/libs/fftShift.cpp:42-61) in place on a device
+ * nx x ny array of element_bytes = 4 (float) or 8 (complex float) elements:
+ * (iy, ix) for iy < ny/2 swaps with ((iy+ny/2)%ny, (ix+nx/2)%nx). */
+int imresh_fft_shift(void *data_dev, uint32_t nx, uint32_t ny, uint32_t element_bytes, void *stream);
 /* Picks (once per device and size, by timing) the row- and column-pass
  * kernel variants used for W x H and reports them.  Called implicitly by the
  * first reconstruction of a size; no reference counterpart (cuFFT planned

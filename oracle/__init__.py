@@ -64,6 +64,8 @@ def _declare(lib):
             f = getattr(lib, f"{name}_{s}")
             f.restype = res
             f.argtypes = args
+    lib.orc_fft_shift.restype = None
+    lib.orc_fft_shift.argtypes = [c_void_p, c_uint, c_uint, c_uint]
     lib.orc_sigma_schedule.restype = None
     lib.orc_sigma_schedule.argtypes = [c_float, c_float, c_uint, c_void_p]
     lib.orc_num_threads.restype = c_int
@@ -282,6 +284,16 @@ def hio_standalone(modulus, mask, n_cycles=0, target_error=1e-6, beta=0.9, dtype
     if rc != 0:
         raise RuntimeError(f"orc_hio_standalone returned {rc}")
     return io, st
+
+
+def fft_shift(a):
+    """libs::fftShift (fftShift.cpp:42-61) of a float32 or complex64 H x W array (a copy)."""
+    x = np.array(a, copy=True, order="C")
+    if x.dtype not in (np.float32, np.complex64):
+        raise ValueError("float32 or complex64")
+    h, w = x.shape
+    lib().orc_fft_shift(x.ctypes.data_as(c_void_p), w, h, x.itemsize)
+    return x
 
 
 def sigma_schedule(sigma0, sigma_change, n):

@@ -66,3 +66,20 @@ void orc_sigma_schedule(float sigma0, float sigma_change, unsigned n, float *out
 #undef T
 #undef S
 #undef SQRT
+
+/* fftShift (src/imresh/libs/fftShift.cpp:42-61): for iy < Ny/2 and every ix,
+ * swap element (iy, ix) with ((iy + Ny/2) % Ny, (ix + Nx/2) % Nx); elements
+ * of elem_bytes bytes (4: float, 8: complex float) */
+void orc_fft_shift(void *data, unsigned nx, unsigned ny, unsigned elem_bytes)
+{
+    unsigned char *d = (unsigned char *) data;
+    unsigned char tmp[16];
+    for (unsigned iy = 0; iy < ny / 2; ++iy)
+        for (unsigned ix = 0; ix < nx; ++ix) {
+            const size_t a = (size_t) iy * nx + ix;
+            const size_t b = (size_t) ((iy + ny / 2) % ny) * nx + (ix + nx / 2) % nx;
+            memcpy(tmp, d + a * elem_bytes, elem_bytes);
+            memcpy(d + a * elem_bytes, d + b * elem_bytes, elem_bytes);
+            memcpy(d + b * elem_bytes, tmp, elem_bytes);
+        }
+}

@@ -15,6 +15,7 @@
  *   calcGaussianKernel    src/imresh/libs/calcGaussianKernel.cpp:42-91
  *   calculateHioError     src/imresh/libs/hybridInputOutput.cpp:67-103
  *   hybridInputOutput     src/imresh/libs/hybridInputOutput.cpp:126-306
+ *   fftShift              src/imresh/libs/fftShift.cpp:42-61
  *   CUDA kernels          src/imresh/algorithms/cuda/cudaVectorElementwise.cu:42-219
  * The FFT (FFTW3f in the reference, absent here) is replaced by a
  * project-written mixed-radix + Bluestein CPU FFT, unnormalized, with
@@ -76,6 +77,9 @@ typedef struct orc_sw_stats {
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)
+
+/* fftShift (fftShift.cpp:42-61) on elements of elem_bytes (4 or 8) */
+void orc_fft_shift(void *data, unsigned nx, unsigned ny, unsigned elem_bytes);
 
 /* sigma schedule of the reference: sigma <- max(1.5, (1 - dsigma) * sigma),
  * in float (shrinkWrap.cpp:245, cudaShrinkWrap.cu:294).  Writes n values. */

@@ -329,3 +329,15 @@ def test_fft_unsupported_size_is_an_error():
     assert L().imresh_fft_supported(2048) == 1 and L().imresh_fft_supported(7) == 1
     assert L().imresh_fft_supported(4096) == 1 and L().imresh_fft_supported(8192) == 1
     assert L().imresh_fft_supported(4097) == 0 and L().imresh_fft_supported(0) == 0
+
+
+@pytest.mark.parametrize("h,w", [(8, 8), (6, 10), (5, 3), (1, 7), (7, 1), (64, 96), (97, 131)])
+def test_fft_shift_matches_oracle(h, w):
+    """fftShift.cpp:42-61, float and complex elements, bit-exact."""
+    rng = np.random.default_rng(h + 100 * w)
+    for a in (rng.standard_normal((h, w)).astype(np.float32),
+              (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)):
+        d = B.from_array(a)
+        im.check(L().imresh_fft_shift(d.ptr, w, h, a.itemsize, None))
+        assert np.array_equal(d.download(a.dtype, a.shape), oracle.fft_shift(a))
+    assert L().imresh_fft_shift(d.ptr, w, h, 3, None) == 2   # element size not 4 or 8

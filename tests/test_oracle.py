@@ -407,3 +407,29 @@ def test_hio_standalone_early_exit_and_defaults():
     a, _ = oracle.hio_standalone(M, mask, n_cycles=2, target_error=0, beta=0, dtype=np.float64)
     b, _ = oracle.hio_standalone(M, mask, n_cycles=2, target_error=0, beta=0.9, dtype=np.float64)
     assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- fftShift
+@pytest.mark.parametrize("h,w", [(4, 6), (8, 8), (6, 10)])
+def test_fft_shift_is_numpy_fftshift_for_even_sizes(h, w):
+    """fftShift.cpp:42-61 swaps (iy, ix) with ((iy+Ny/2)%Ny, (ix+Nx/2)%Nx):
+    for even sizes that is numpy's fftshift, and an involution."""
+    rng = np.random.default_rng(h * w)
+    a = rng.standard_normal((h, w)).astype(np.float32)
+    assert np.array_equal(oracle.fft_shift(a), np.fft.fftshift(a))
+    c = (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)
+    assert np.array_equal(oracle.fft_shift(c), np.fft.fftshift(c))
+    assert np.array_equal(oracle.fft_shift(oracle.fft_shift(c)), c)
+
+
+def test_fft_shift_odd_sizes_brute_force():
+    """Odd sizes: the reference loop (rows iy < Ny/2 only) is not numpy's
+    fftshift; checked against a direct Python transcription on tiny inputs."""
+    for h, w in [(5, 3), (3, 4), (1, 7), (7, 1)]:
+        a = np.arange(h * w, dtype=np.float32).reshape(h, w)
+        exp = a.copy().ravel()
+        for iy in range(h // 2):
+            for ix in range(w):
+                i, j = iy * w + ix, ((iy + h // 2) % h) * w + (ix + w // 2) % w
+                exp[i], exp[j] = exp[j], exp[i]
+        assert np.array_equal(oracle.fft_shift(a).ravel(), exp), (h, w)
