@@ -535,6 +535,7 @@ __global__ void __launch_bounds__(256) k_fft_shift(E *d, unsigned nx, unsigned n
 int launch_fft_shift(void *d, unsigned nx, unsigned ny, unsigned elem_bytes, hipStream_t s)
 {
     const size_t half = (size_t) (ny / 2) * nx;
+    if (elem_bytes != 4 && elem_bytes != 8) return IMRESH_ERR_INVALID;
     if (half == 0) return IMRESH_OK;
     if (elem_bytes == 4)
         hipLaunchKernelGGL(k_fft_shift<float>, dim3(grid_1d(half)), dim3(256), 0, s, (float *) d, nx, ny);
