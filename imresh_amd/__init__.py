@@ -124,8 +124,12 @@ _SIGS = {
     "imresh_gaussian_blur_vertical": (c_int, [_P, c_uint32, c_uint32, c_double, _P]),
     "imresh_fft2d": (c_int, [_P, _P, c_uint32, c_uint32, c_int, _P]),
     "imresh_fft_supported": (c_int, [c_uint32]),
+    "imresh_diffraction_intensity": (c_int, [_P, c_uint32, c_uint32, _P, c_int]),
+    "imresh_diffraction_intensity_device": (c_int, [_P, c_uint32, c_uint32, _P]),
     "imresh_fft_shift": (c_int, [_P, c_uint32, c_uint32, c_uint32, _P]),
     "imresh_autotune": (c_int, [c_uint32, c_uint32, _P, POINTER(c_int), POINTER(c_int)]),
+    "imresh_variant_counts": (c_int, [c_uint32, c_uint32, POINTER(c_int), POINTER(c_int)]),
+    "imresh_set_variants": (c_int, [c_uint32, c_uint32, c_int, c_int]),
     "imresh_task_queue_init": (c_int, [c_int]),
     "imresh_task_queue_deinit": (c_int, []),
     "imresh_add_task": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), TASK_CALLBACK, _P]),
@@ -317,6 +321,18 @@ def hybrid_input_output(io: np.ndarray, mask: np.ndarray, n_cycles: int = 0,
                                             target_error, hio_beta, stream, ctypes.byref(r)),
           "imresh_hybrid_input_output")
     return r
+
+
+def diffraction_intensity(io: np.ndarray, stream=None) -> np.ndarray:
+    """imresh::libs::diffractionIntensity on the GPU: `io` (float32 H x W,
+    the real object) is overwritten with |FFT2(io)| (unnormalized, frequency 0
+    at [0, 0]) and returned."""
+    if io.dtype != np.float32 or not io.flags.c_contiguous or io.ndim != 2:
+        raise ValueError("io must be a C-contiguous float32 H x W array")
+    h, w = io.shape
+    check(_lib().imresh_diffraction_intensity(io.ctypes.data, w, h, stream, 0),
+          "imresh_diffraction_intensity")
+    return io
 
 
 class Server:

@@ -933,6 +933,48 @@ int hio_standalone_entry(float *io, const uint8_t *mask, uint32_t W, uint32_t H,
     return st;
 }
 
+// Forward model |FFT2((x, 0))| (diffractionIntensity.cpp:49-80, CUDA
+// :84-135): the column pass reads the real plane and writes the complex
+// half-transform into the workspace, the row pass finishes the transform and
+// stores only |z|, in place over the input (24 B/element of HBM traffic; the
+// reference's path moves 4+8 (H2D), 8 (memset), cuFFT 2 x 16, 8+4 (norm), 4
+// (D2H)).  Host io goes through the workspace's M plane.
+int diffraction_entry(float *io, uint32_t W, uint32_t H, void *stream, bool device_io, bool async)
+{
+    if (io == nullptr) return IMRESH_ERR_NULL;
+    if (W == 0 || H == 0) return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, W, H, &p));
+    hipStream_t s = S(stream);
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, W, H, s, &w));
+    const size_t n = w->n;
+    float *plane = device_io ? io : w->M;
+    int st = IMRESH_OK;
+    if (!device_io) st = from_hip(hipMemcpyAsync(plane, io, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    if (st == IMRESH_OK) {
+        ColArgs ca{};
+        ca.M = plane;
+        ca.out = w->X;
+        ca.W = (int) W;
+        st = launch_col(COL_REAL_FWD, p, ca, s);
+    }
+    if (st == IMRESH_OK) {
+        RowArgs ra{};
+        ra.in = w->X;
+        ra.Bn = plane;
+        ra.H = (int) H;
+        st = launch_row(ROW_FWD_NORM, p, ra, s);
+    }
+    if (st == IMRESH_OK && !device_io)
+        st = from_hip(hipMemcpyAsync(io, plane, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (st == IMRESH_OK && !async) st = from_hip(hipStreamSynchronize(s));
+    ws_return(w, s);
+    return st;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1090,10 +1132,44 @@ int imresh_fft2d(float *out, const float *in, uint32_t W, uint32_t H, int direct
 
 int imresh_fft_supported(uint32_t n) { return line_supported((int) n) ? 1 : 0; }
 
+int imresh_diffraction_intensity(float *io, uint32_t width, uint32_t height, void *stream, int async)
+{
+    return diffraction_entry(io, width, height, stream, false, async != 0);
+}
+
+int imresh_diffraction_intensity_device(float *io_dev, uint32_t width, uint32_t height, void *stream)
+{
+    return diffraction_entry(io_dev, width, height, stream, true, true);
+}
+
 int imresh_fft_shift(void *data_dev, uint32_t nx, uint32_t ny, uint32_t element_bytes, void *stream)
 {
     if (data_dev == nullptr) return IMRESH_ERR_NULL;
     return launch_fft_shift(data_dev, nx, ny, element_bytes, S(stream));
+}
+
+int imresh_variant_counts(uint32_t width, uint32_t height, int *row_variants_out, int *col_variants_out)
+{
+    if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, width, height, &p));
+    if (row_variants_out) *row_variants_out = std::max(row_variants(p.row), 1);
+    if (col_variants_out) *col_variants_out = std::max(col_variants(p.col), 1);
+    return IMRESH_OK;
+}
+
+int imresh_set_variants(uint32_t width, uint32_t height, int row_variant, int col_variant)
+{
+    int nr, nc;
+    IMR_CHECK(imresh_variant_counts(width, height, &nr, &nc));
+    if (row_variant < 0 || row_variant >= nr || col_variant < 0 || col_variant >= nc)
+        return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    set_tuned_variants(dev, (int) width, (int) height, row_variant, col_variant);
+    return IMRESH_OK;
 }
 
 int imresh_autotune(uint32_t width, uint32_t height, void *stream, int *row_variant,

@@ -431,6 +431,20 @@ __device__ __forceinline__ void line_fft(cf (&v)[P::E], const cf *tw, int t, con
 // stalling the junction; a `pre` returning NoPre leaves the loads to `mid`.
 struct NoPre {};
 
+// the pair after its input gather: v holds the gathered line, pf the
+// prefetched junction operands
+template <class P, int DA, int DB, class Lds, class PT, class Mid, class Dst>
+__device__ __forceinline__ void line_pair_loaded(cf (&v)[P::E], PT (&pf)[P::E], const cf *tw, int t,
+                                                 const Lds &L, Mid &mid, Dst &dst)
+{
+    compute<P, DA, 0>(v, tw, t);
+    passes_from<P, DA, 1>(v, tw, t, L);
+    for_outputs<P, DA, P::np - 1>(t, [&](int s, int e) { v[s] = mid(e, v[s], pf[s]); });
+    compute<P, DB, 0>(v, tw, t);
+    passes_from<P, DB, 1>(v, tw, t, L);
+    scatter_global<P, DB, P::np - 1>(v, t, dst);
+}
+
 template <class P, int DA, int DB, class Lds, class Src, class Pre, class Mid, class Dst>
 __device__ __forceinline__ void line_pair(cf (&v)[P::E], const cf *tw, int t, const Lds &L, Src &src,
                                           Pre &pre, Mid &mid, Dst &dst)
@@ -441,12 +455,7 @@ __device__ __forceinline__ void line_pair(cf (&v)[P::E], const cf *tw, int t, co
     PT pf[P::E];
     gather_global<P, DA, 0>(v, t, src);
     for_outputs<P, DA, P::np - 1>(t, [&](int s, int e) { pf[s] = pre(e); });
-    compute<P, DA, 0>(v, tw, t);
-    passes_from<P, DA, 1>(v, tw, t, L);
-    for_outputs<P, DA, P::np - 1>(t, [&](int s, int e) { v[s] = mid(e, v[s], pf[s]); });
-    compute<P, DB, 0>(v, tw, t);
-    passes_from<P, DB, 1>(v, tw, t, L);
-    scatter_global<P, DB, P::np - 1>(v, t, dst);
+    line_pair_loaded<P, DA, DB>(v, pf, tw, t, L, mid, dst);
 }
 
 // ------------------------------------------------------------ reductions

@@ -21,6 +21,7 @@ enum RowMode {
     ROW_LAST,           // X -> inverse -> X = g', Bn = |g'|, error partials
     ROW_LAST_PLAIN,     // X -> inverse -> X = g'
     ROW_INIT_ERR,       // ROW_INIT + error partials
+    ROW_FWD_NORM,       // in -> forward row transform -> Bn = |z| (float)
     ROW_NMODES
 };
 
@@ -29,6 +30,7 @@ enum ColMode {
     COL_INV,            // in -> out, inverse column transform
     COL_PROJ,           // X -> forward -> modulus projection -> inverse -> X
     COL_INIT,           // real M -> inverse -> X
+    COL_REAL_FWD,       // real M -> forward -> out
     COL_NMODES
 };
 
@@ -38,7 +40,7 @@ struct RowArgs {
     float2 *X;
     float2 *G;
     uint8_t *mask;
-    float *Bn;
+    float *Bn;             // |g'| (RK_INVOUT) or |z| (ROW_FWD_NORM)
     const float *B2;
     const float *thr;      // device scalar: mask threshold
     float beta;

@@ -9,7 +9,9 @@
 //   CK_INV   inverse column transform; with kColRealIn the input is the real
 //            modulus M, i.e. the first half of IFFT2((M, 0))
 //            (cudaShrinkWrap.cu:229-232).
-//   CK_FWD   forward column transform of the 2-D FFT entry point.
+//   CK_FWD   forward column transform of the 2-D FFT entry point; with
+//            kColRealIn the input is a real plane, the first half of the
+//            forward model FFT2((x, 0)) (diffractionIntensity.cpp:59-70).
 #include "internal.hpp"
 #include "blue_device.hpp"
 #include "ops_device.hpp"
@@ -66,9 +68,11 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
     const Blue bl{BL ? a.len : P::n, (const cf *) a.bs};
 
     if constexpr (K == CK_FWD) {
+        const bool real_in = (a.flags & kColRealIn) != 0;
         const cf *in = (const cf *) a.in + wb;
+        const float *M = a.M + wb;
         cf *out = (cf *) a.out + wb;
-        auto g0 = [=](int e) { return at32(in, e * W + co); };
+        auto g0 = [=](int e) { return real_in ? mkc(at32(M, e * W + co), 0.f) : at32(in, e * W + co); };
         auto sl = [=](int e, cf x) { if (valid) at32(out, e * W + co) = x; };
         xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
     } else if constexpr (K == CK_INV) {
@@ -122,6 +126,11 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
     switch (mode) {
     case COL_FWD:
         a.flags = 0;
+        IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
+        break;
+    case COL_REAL_FWD:
+        a.flags = kColRealIn;
+        a.in = a.out;   // unused with kColRealIn; never a null pointer in a kernel
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INV:

@@ -13,7 +13,8 @@
 //   RK_INVOUT inverse row FFT closing a reconstruction step: writes g' and,
 //             by flags, gPrev = g' (cudaShrinkWrap.cu:304-309), |g'| for the
 //             blur (:240) and the HIO error sums (cudaVectorReduce.cu:284-355).
-//   RK_FWD / RK_INV  plain row transforms of the 2-D FFT entry point.
+//   RK_FWD / RK_INV  plain row transforms of the 2-D FFT entry point; with
+//            kRowNormOut the forward one stores |z| (the forward model).
 #include "internal.hpp"
 #include "blue_device.hpp"
 #include "ops_device.hpp"
@@ -66,9 +67,21 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         const cf *in = (const cf *) a.in + wb;
         cf *out = (cf *) a.out + wb;
         auto g0 = [=](int e) { return at32(in, lo + e); };
-        auto sl = [=](int e, cf x) { if (valid) at32(out, lo + e) = x; };
-        if constexpr (K == RK_FWD) xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
-        else xfft<P, +1, BL>(v, tw, t, L, bl, g0, sl);
+        if constexpr (K == RK_FWD) {
+            // kRowNormOut: the last half of the forward model |FFT2(x)|
+            // (diffractionIntensity.cpp:72-77), only |z| leaves the registers
+            const bool norm = (a.flags & kRowNormOut) != 0;
+            float *Bn = a.Bn + wb;
+            auto sl = [=](int e, cf x) {
+                if (!valid) return;
+                if (norm) at32(Bn, lo + e) = cnorm(x);
+                else at32(out, lo + e) = x;
+            };
+            xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
+        } else {
+            auto sl = [=](int e, cf x) { if (valid) at32(out, lo + e) = x; };
+            xfft<P, +1, BL>(v, tw, t, L, bl, g0, sl);
+        }
     } else if constexpr (K == RK_INVOUT) {
         __shared__ float red_f[NL * P::T];
         __shared__ unsigned red_u[NL * P::T];
@@ -167,6 +180,11 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
     }
     switch (mode) {
     case ROW_FWD: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
+    case ROW_FWD_NORM:
+        a.flags = kRowNormOut;
+        a.out = (float2 *) a.Bn;   // unused with kRowNormOut; never a null pointer in a kernel
+        IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a);
+        break;
     case ROW_INV: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_INV, false, BL>), grid, block, lds, s, a); break;
     case ROW_INIT:
         a.flags = kRowWriteG | kRowWriteBn;

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "imresh/algorithms/shrinkWrap.hpp"
+#include "imresh/libs/diffractionIntensity.hpp"
 #include "imresh/libs/fftShift.hpp"
 #include "imresh/libs/hybridInputOutput.hpp"
 #include "imresh/io/taskQueue.hpp"
@@ -227,6 +228,14 @@ int hybridInputOutput(float *const rIoData, uint8_t const *const rIsMasked,
     const int st = imresh_hybrid_input_output(rIoData, rIsMasked, rSize[0], rSize[1], rnCycles,
                                               rTargetError, rBeta, nullptr, nullptr);
     return st == IMRESH_OK ? 0 : st;
+}
+
+void diffractionIntensity(float *rIoData, unsigned int rImageWidth, unsigned int rImageHeight,
+                          hipStream_t rStream, bool rAsync)
+{
+    // the reference exits on a CUDA or cuFFT error; here the status is dropped
+    // as the signature has no place for it (the C entry point reports it)
+    imresh_diffraction_intensity(rIoData, rImageWidth, rImageHeight, (void *) rStream, rAsync ? 1 : 0);
 }
 
 template <class T_COMPLEX>

@@ -258,6 +258,15 @@ int imresh_gaussian_blur_vertical(float *data_dev, uint32_t width, uint32_t heig
  * out may equal in. */
 int imresh_fft2d(float *out_dev, const float *in_dev, uint32_t width, uint32_t height,
                  int direction, void *stream);
+/* Forward model (src/imresh/libs/diffractionIntensity.cpp:49-80, CUDA
+ * :84-135): io (a width x height real object, row-major) is replaced in place
+ * by |FFT2((x, 0))|, the unnormalized forward transform's modulus with
+ * frequency 0 at [0, 0], i.e. the modulus input of imresh_shrinkwrap.
+ * imresh_diffraction_intensity takes host memory and synchronizes the stream
+ * unless async != 0 (then io is valid once the stream has finished);
+ * the _device variant takes device memory and is asynchronous. */
+int imresh_diffraction_intensity(float *io, uint32_t width, uint32_t height, void *stream, int async);
+int imresh_diffraction_intensity_device(float *io_dev, uint32_t width, uint32_t height, void *stream);
 /* 1 if an FFT plan exists for this line length, else 0. */
 int imresh_fft_supported(uint32_t n);
 /* fftShift (src/imresh/libs/fftShift.cpp:42-61) in place on a device
@@ -270,6 +279,13 @@ int imresh_fft_shift(void *data_dev, uint32_t nx, uint32_t ny, uint32_t element_
  * internally, cudaShrinkWrap.cu:226). */
 int imresh_autotune(uint32_t width, uint32_t height, void *stream, int *row_variant,
                     int *col_variant);
+/* Number of row- and column-pass kernel variants of W x H (every variant
+ * computes the same bits; they differ in tiling and scheduling). */
+int imresh_variant_counts(uint32_t width, uint32_t height, int *row_variants, int *col_variants);
+/* Pins the variants used for W x H on the current device instead of timing
+ * them (a known tuning, or a test of one variant).  No reference
+ * counterpart. */
+int imresh_set_variants(uint32_t width, uint32_t height, int row_variant, int col_variant);
 
 /* ---- task queue (taskQueue.hpp:38-84, taskQueue.cu:91-251) --------------- */
 /* Completion callback: (user, io_host, width, height, status). */

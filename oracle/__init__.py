@@ -56,6 +56,7 @@ def _declare(lib):
             "orc_hio_iterations": (c_int, [P, P, P, P, c_uint, c_uint, ct, c_uint]),
             "orc_shrinkwrap": (c_int, [P, c_uint, c_uint, c_uint, c_float, c_float, c_float,
                                        c_float, c_float, c_float, c_uint, POINTER(SwStats)]),
+            "orc_diffraction_intensity": (c_int, [P, c_uint, c_uint]),
             "orc_random_phase": (None, [P, P, c_uint, c_uint]),
             "orc_hio_standalone": (c_int, [P, P, c_uint, c_uint, c_uint, c_float, c_float,
                                            POINTER(SwStats)]),
@@ -261,6 +262,17 @@ def shrink_wrap(modulus, n_cycles=20, target_error=1e-5, hio_beta=0.9, cutoff_au
     if rc != 0:
         raise RuntimeError(f"orc_shrinkwrap returned {rc}")
     return io, st
+
+
+def diffraction_intensity(x, dtype=np.float32):
+    """libs::diffractionIntensity (diffractionIntensity.cpp:49-80): |FFT2(x)|
+    of a real H x W object, unnormalized, frequency 0 at [0, 0] (a copy)."""
+    io = np.array(x, dtype=dtype, copy=True, order="C")
+    h, w = io.shape
+    rc = getattr(lib(), f"orc_diffraction_intensity_{_sfx(dtype)}")(_p(io), w, h)
+    if rc != 0:
+        raise RuntimeError(f"orc_diffraction_intensity returned {rc}")
+    return io
 
 
 def random_phase(norm, dtype=np.float32):

@@ -70,6 +70,7 @@ typedef struct orc_sw_stats {
                             float cutoff_autocorel, float cutoff, float sigma0,    \
                             float sigma_change, unsigned n_hio,                    \
                             orc_sw_stats *stats);                                  \
+    int  orc_diffraction_intensity_##S(T *io, unsigned w, unsigned h);             \
     void orc_random_phase_##S(const T *norm, T *out, unsigned w, unsigned h);      \
     int  orc_hio_standalone_##S(T *io, const unsigned char *mask, unsigned w,      \
                                 unsigned h, unsigned n_cycles, float target,       \

@@ -473,6 +473,29 @@ int FN(orc_shrinkwrap)(T *io, unsigned w, unsigned h, unsigned n_cycles,
 }
 
 /* ------------------------------------------------------------------------ */
+/* forward model (diffractionIntensity.cpp:49-80, CUDA :84-135)              */
+/* ------------------------------------------------------------------------ */
+/* io (real object) <- |FFT2((x, 0))|: the real plane copied into the real
+ * parts with zero imaginary parts (:59-63 / the memset + strided copy of
+ * :98-113), the unnormalized FORWARD 2-D transform of H rows x W columns
+ * (:64-68 / :114-118) and sqrt(re*re + im*im) per element (:72-77 /
+ * cudaComplexNormElementwise), no fftshift */
+int FN(orc_diffraction_intensity)(T *io, unsigned w, unsigned h)
+{
+    const size_t n = (size_t) w * h;
+    T *tmp = (T *) malloc(sizeof(T) * 2 * n);
+    if (tmp == NULL) return 1;
+    for (size_t i = 0; i < n; ++i) {
+        tmp[2 * i] = io[i];
+        tmp[2 * i + 1] = 0;
+    }
+    const int rc = FN(orc_fft2d)(tmp, tmp, w, h, -1);
+    if (rc == 0) FN(orc_complex_norm)(io, tmp, (unsigned) n);
+    free(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* standalone HIO with a given support (hybridInputOutput.cpp:126-306)       */
 /* ------------------------------------------------------------------------ */
 /* addRandomPhase (:126-197): srand(2623091912); a real field rand()/RAND_MAX,

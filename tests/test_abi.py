@@ -44,7 +44,10 @@ def test_exports_cpp_dropin_api():
                 "float, float, float, float, float, float, unsigned int)",
                 "imresh::io::taskQueueInit()", "imresh::io::taskQueueDeinit()",
                 "imresh::io::addTask(",
-                "imresh::libs::hybridInputOutput(float*, unsigned char const*, std::vector<unsigned int"):
+                "imresh::libs::hybridInputOutput(float*, unsigned char const*, std::vector<unsigned int",
+                "imresh::libs::diffractionIntensity(float*, unsigned int, unsigned int, ihipStream_t*, bool)",
+                "imresh::libs::fftShift<float>(float*, unsigned int, unsigned int)",
+                "imresh::libs::fftShift<float [2]>(float (*) [2], unsigned int, unsigned int)"):
         assert sym in out, sym
 
 

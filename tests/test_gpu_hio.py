@@ -46,6 +46,31 @@ def _gpu_iterations(M, g, gp, mask, beta, n_iter):
     return dg.download(np.complex64, (h, w)), dgp.download(np.complex64, (h, w))
 
 
+@pytest.mark.parametrize("w,h", [(2048, 2048), (1024, 1024), (4096, 256), (256, 2048)])
+def test_every_kernel_variant_computes_the_same_bits(w, h):
+    """The autotuner picks among row/column kernel variants that differ only
+    in tiling and scheduling (threads per line, lines per workgroup,
+    prefetching, staggered starts, sequential sub-tiles): each must give the
+    bits of variant 0 over two fused iterations."""
+    obj, M, g, gp, mask = _state(w, h, index=3)
+    nr, nc = ctypes.c_int(), ctypes.c_int()
+    im.check(L().imresh_variant_counts(w, h, ctypes.byref(nr), ctypes.byref(nc)))
+    rv0, cv0 = ctypes.c_int(), ctypes.c_int()
+    im.check(L().imresh_autotune(w, h, None, ctypes.byref(rv0), ctypes.byref(cv0)))
+    try:
+        im.check(L().imresh_set_variants(w, h, 0, 0))
+        ref = _gpu_iterations(M, g, gp, mask, 0.9, 2)
+        for rv, cv in [(r, 0) for r in range(1, nr.value)] + [(0, c) for c in range(1, nc.value)]:
+            im.check(L().imresh_set_variants(w, h, rv, cv))
+            got = _gpu_iterations(M, g, gp, mask, 0.9, 2)
+            assert np.array_equal(got[0].view(np.float32), ref[0].view(np.float32)), (rv, cv)
+            assert np.array_equal(got[1].view(np.float32), ref[1].view(np.float32)), (rv, cv)
+        assert L().imresh_set_variants(w, h, nr.value, 0) == 2
+        assert L().imresh_set_variants(w, h, 0, nc.value) == 2
+    finally:
+        im.check(L().imresh_set_variants(w, h, rv0.value, cv0.value))
+
+
 @pytest.mark.parametrize("w,h", [(64, 64), (256, 256), (96, 200), (1024, 1024), (2048, 2048),
                                  (1280, 1536), (640, 3072)])
 def test_one_hio_iteration_matches_oracle(w, h):

@@ -13,6 +13,7 @@ import pytest
 
 import imresh_amd as im
 import oracle
+from imresh_amd import synthetic as syn
 from helpers import FLT_EPS, KAT_SEED, RAND_MAX, compare_float_array, log_spaced_points, rand, srand
 
 pytestmark = pytest.mark.gpu
@@ -341,3 +342,61 @@ def test_fft_shift_matches_oracle(h, w):
         im.check(L().imresh_fft_shift(d.ptr, w, h, a.itemsize, None))
         assert np.array_equal(d.download(a.dtype, a.shape), oracle.fft_shift(a))
     assert L().imresh_fft_shift(d.ptr, w, h, 3, None) == 2   # element size not 4 or 8
+
+
+# ---------------------------------------------------------------- forward model
+DIFF_SIZES = [(64, 64), (256, 256), (2048, 2048), (64, 96), (97, 131), (300, 200),
+              (1280, 1536), (8, 8192), (1, 7)]
+
+
+@pytest.mark.parametrize("h,w", DIFF_SIZES)
+def test_diffraction_intensity_vs_oracle(h, w):
+    """imresh_diffraction_intensity (diffractionIntensity.cpp:84-135) against
+    the float64 oracle: relative L2 error within the fp32 FFT bound
+    2e-7 * log2(N); the host and device entry points agree bit for bit."""
+    rng = np.random.default_rng(h * 13 + w)
+    x = rng.random((h, w)).astype(np.float32)
+    exp = oracle.diffraction_intensity(x.astype(np.float64), dtype=np.float64)
+    got = im.diffraction_intensity(x.copy())
+    tol = 2e-7 * max(1.0, np.log2(h * w))
+    assert np.linalg.norm(got - exp) / np.linalg.norm(exp) < tol
+    d = B.from_array(x)
+    im.check(L().imresh_diffraction_intensity_device(d.ptr, w, h, None))
+    assert np.array_equal(d.download(np.float32, (h, w)), got)
+    y = x.copy()
+    im.check(L().imresh_diffraction_intensity(y.ctypes.data, w, h, None, 1))   # async
+    im.sync()
+    assert np.array_equal(y, got)
+
+
+def test_diffraction_intensity_closed_forms_and_errors():
+    h, w = 64, 128
+    d = np.zeros((h, w), np.float32)
+    d[0, 0] = 1   # every twiddle product is exact: all ones
+    assert np.array_equal(im.diffraction_intensity(d.copy()), np.ones((h, w), np.float32))
+    d[0, 0] = 0
+    d[3, 5] = 1   # |exp(i phi)| rounded in fp32
+    assert np.max(np.abs(im.diffraction_intensity(d.copy()) - 1)) <= 4 * FLT_EPS
+    z = np.zeros((h, w), np.float32)
+    assert np.array_equal(im.diffraction_intensity(z), z)
+    assert L().imresh_diffraction_intensity(None, w, h, None, 0) == 1           # IMRESH_ERR_NULL
+    assert L().imresh_diffraction_intensity(d.ctypes.data, 0, h, None, 0) == 2  # IMRESH_ERR_INVALID
+    assert L().imresh_diffraction_intensity(d.ctypes.data, 4097, 1, None, 0) == 3
+
+
+def test_device_forward_model_feeds_reconstruction():
+    """Object -> forward model -> shrink-wrap entirely in device memory gives
+    the same result as the host path (forward model on the host buffer, then
+    imresh_shrinkwrap), bit for bit."""
+    obj, _ = syn.workload(128, 96, n_shapes=5, central=48, index=4)
+    p = im.default_params(n_cycles=3, n_hio_cycles=4)
+    host = obj.astype(np.float32).copy()
+    im.diffraction_intensity(host)
+    r = im.Result()
+    im.check(L().imresh_shrinkwrap(host.ctypes.data, 128, 96, p, None, r))
+    d = B.from_array(obj.astype(np.float32))
+    im.check(L().imresh_diffraction_intensity_device(d.ptr, 128, 96, None))
+    r2 = im.Result()
+    im.check(L().imresh_shrinkwrap_device(d.ptr, 128, 96, p, None, r2))
+    assert np.array_equal(d.download(np.float32, (96, 128)), host)
+    assert r.cycles_run == r2.cycles_run and r.last_error == r2.last_error

@@ -433,3 +433,49 @@ def test_fft_shift_odd_sizes_brute_force():
                 i, j = iy * w + ix, ((iy + h // 2) % h) * w + (ix + w // 2) % w
                 exp[i], exp[j] = exp[j], exp[i]
         assert np.array_equal(oracle.fft_shift(a).ravel(), exp), (h, w)
+
+
+# ---------------------------------------------------------------- forward model
+@pytest.mark.parametrize("h,w", [(8, 8), (6, 10), (5, 3), (1, 7), (12, 1), (64, 96), (97, 131)])
+def test_diffraction_intensity_vs_numpy(h, w):
+    """diffractionIntensity.cpp:49-80 is |FFT2((x, 0))| in H rows x W
+    columns without a shift: numpy's |fft2| of the real array (non-square and
+    odd shapes fail a transposed or shifted layout)."""
+    rng = np.random.default_rng(h * 31 + w)
+    x = rng.random((h, w))
+    got = oracle.diffraction_intensity(x, dtype=np.float64)
+    exp = np.abs(np.fft.fft2(x))
+    assert np.max(np.abs(got - exp)) <= 1e-12 * max(1.0, exp.max())
+    g32 = oracle.diffraction_intensity(x.astype(np.float32))
+    assert g32.dtype == np.float32
+    assert np.linalg.norm(g32 - exp) / np.linalg.norm(exp) < 2e-7 * max(1.0, np.log2(h * w))
+
+
+def test_diffraction_intensity_closed_forms():
+    h, w = 48, 40
+    # a point object anywhere has a flat diffraction pattern |F| = value
+    d = np.zeros((h, w))
+    d[7, 13] = 2.5
+    assert np.allclose(oracle.diffraction_intensity(d, dtype=np.float64), 2.5, rtol=0, atol=1e-12)
+    # an a x b box at the origin: product of two Dirichlet kernels
+    a, b = 5, 9
+    box = np.zeros((h, w))
+    box[:a, :b] = 1
+    ky = np.arange(h)[:, None]
+    kx = np.arange(w)[None, :]
+
+    def dirichlet(k, n, m):
+        s = np.sin(np.pi * k / n)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            r = np.abs(np.sin(np.pi * m * k / n) / s)
+        return np.where(k % n == 0, float(m), r)
+
+    exp = dirichlet(ky, h, a) * dirichlet(kx, w, b)
+    got = oracle.diffraction_intensity(box, dtype=np.float64)
+    assert np.max(np.abs(got - exp)) < 1e-10
+    # Parseval: sum |F|^2 = N sum x^2, and |F| is centro-symmetric for real x
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((h, w))
+    F = oracle.diffraction_intensity(x, dtype=np.float64)
+    assert abs((F ** 2).sum() - h * w * (x ** 2).sum()) < 1e-9 * h * w * (x ** 2).sum()
+    assert np.allclose(F, np.roll(F[::-1, ::-1], (1, 1), axis=(0, 1)), rtol=1e-12, atol=1e-12)
