@@ -21,6 +21,8 @@ Prints ONE JSON line (the benchmark contract) with two extra objects:
                 single-threaded FFT as the reference's FFTW plans, OpenMP
                 elementwise loops) timed on a bounded sample of HIO
                 iterations of the same workload
+Further legs (batched, serving, 8192^2, 2-D FFT vs rocFFT, forward model)
+are reported beside them; --no-<leg> skips one.
 """
 from __future__ import annotations
 
@@ -317,6 +319,44 @@ def large_bench(L, M8, steps: int = 2) -> dict:
             "iteration_alg_GBps_80B": round(80 * n * n * N_CYCLES * N_HIO / dt / 1e9, 1)}
 
 
+def forward_model_bench(L, obj: np.ndarray, reps: int = 50) -> dict:
+    """The device forward model |FFT2(object)| (imresh_diffraction_intensity_device)
+    at the object's size: kernel-exact durations of its two fused passes
+    (real-input column pass 12 B/element, modulus-output row pass 12 B/element),
+    wall-clock rate of back-to-back calls (each preceded by a 16 MiB device
+    copy restoring the object, timed with it) and parity against numpy in
+    float64."""
+    h, w = obj.shape
+    x = np.ascontiguousarray(obj, dtype=np.float32)
+    d_src = im.DeviceBuffer.from_array(x)
+    d_io = im.DeviceBuffer(x.nbytes)
+
+    def step():
+        im.check(L.imresh_memcpy_d2d(d_io.ptr, d_src.ptr, x.nbytes, None), "d2d")
+        im.check(L.imresh_diffraction_intensity_device(d_io.ptr, w, h, None), "forward model")
+
+    step()
+    im.sync()
+    got = d_io.download(np.float32, (h, w))
+    exp = np.abs(np.fft.fft2(x.astype(np.float64)))
+    rel = float(np.linalg.norm(got - exp) / np.linalg.norm(exp))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    im.sync()
+    dt = (time.perf_counter() - t0) / reps
+    ms = (im.ctypes.c_float * 2)()
+    im.check(L.imresh_profile_forward_model(w, h, 20, ms, None), "profile forward model")
+    n = w * h
+    col_ms, row_ms = float(ms[0]), float(ms[1])
+    return {"workload": f"|FFT2(object)| {w}x{h} fp32, object resident in HBM",
+            "images_per_s_with_restore_copy": round(1.0 / dt, 1),
+            "kernels": {"col_real_fwd": {"ms": round(col_ms, 5), "GB/s": round(12 * n / (col_ms * 1e-3) / 1e9, 1)},
+                        "row_fwd_norm": {"ms": round(row_ms, 5), "GB/s": round(12 * n / (row_ms * 1e-3) / 1e9, 1)}},
+            "alg_GBps_24B": round(24 * n / ((col_ms + row_ms) * 1e-3) / 1e9, 1),
+            "rel_l2_vs_numpy_fp64": rel}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +368,7 @@ def main() -> int:
     ap.add_argument("--serve-count", type=int, default=300)
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-fft-compare", action="store_true")
+    ap.add_argument("--no-forward-model", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -413,6 +454,11 @@ def main() -> int:
         large = large_bench(L, M8)
         del M8
 
+    fwd = None
+    if not args.no_forward_model:
+        log("[bench] forward model |FFT2(object)|")
+        fwd = forward_model_bench(L, obj)
+
     fftc = None
     if not args.no_fft_compare:
         from imresh_amd import fft_compare
@@ -434,7 +480,7 @@ def main() -> int:
                    "iterations_per_step": N_CYCLES * N_HIO, "width": W, "height": H,
                    "row_variant": rv.value, "col_variant": cv.value},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batch,
-        "serving": serve, "large_8192": large, "fft_vs_rocfft": fftc,
+        "serving": serve, "large_8192": large, "fft_vs_rocfft": fftc, "forward_model": fwd,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -134,6 +134,7 @@ _SIGS = {
     "imresh_task_queue_deinit": (c_int, []),
     "imresh_add_task": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), TASK_CALLBACK, _P]),
     "imresh_profile_kernels": (c_int, [c_uint32, c_uint32, c_uint32, POINTER(c_float), _P]),
+    "imresh_profile_forward_model": (c_int, [c_uint32, c_uint32, c_uint32, POINTER(c_float), _P]),
 }
 
 

@@ -1264,4 +1264,54 @@ int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms,
     return st;
 }
 
+int imresh_profile_forward_model(uint32_t W, uint32_t H, uint32_t reps, float *out_ms, void *stream)
+{
+    if (out_ms == nullptr) return IMRESH_ERR_NULL;
+    if (W == 0 || H == 0 || reps == 0) return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    hipStream_t s = S(stream);
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, W, H, &p));
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, W, H, s, &w));
+    int st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, W, s);
+    ColArgs ca{};
+    ca.M = w->Bn;   // a filled real plane
+    ca.out = w->X;
+    RowArgs ra{};
+    ra.in = w->X;
+    ra.Bn = w->B2;
+    double acc[2] = {0, 0};
+    int cnt[2] = {0, 0};
+    for (int round = 0; round < 2 && st == IMRESH_OK; ++round) {   // round 0 warms up
+        for (uint32_t i = 0; i < std::max(4u, reps) && st == IMRESH_OK; ++i) {
+            for (int k = 0; k < 2 && st == IMRESH_OK; ++k) {
+                hipEvent_t a = nullptr, b = nullptr;
+                st = from_hip(hipEventCreate(&a));
+                if (st == IMRESH_OK) st = from_hip(hipEventCreate(&b));
+                if (st == IMRESH_OK) {
+                    g_prof.start = a;
+                    g_prof.stop = b;
+                    st = k == 0 ? launch_col(COL_REAL_FWD, p, ca, s) : launch_row(ROW_FWD_NORM, p, ra, s);
+                    g_prof = ProfEvents{};
+                }
+                if (st == IMRESH_OK) st = from_hip(hipEventSynchronize(b));
+                float ms = 0.f;
+                if (st == IMRESH_OK && round == 1 && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+                    acc[k] += ms;
+                    cnt[k] += 1;
+                }
+                if (a) hipEventDestroy(a);
+                if (b) hipEventDestroy(b);
+            }
+        }
+    }
+    hipStreamSynchronize(s);
+    ws_return(w, s);
+    if (st == IMRESH_OK)
+        for (int k = 0; k < 2; ++k) out_ms[k] = cnt[k] ? (float) (acc[k] / cnt[k]) : 0.f;
+    return st;
+}
+
 }  // extern "C"

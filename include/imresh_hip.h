@@ -307,6 +307,11 @@ int imresh_add_task(float *io_host, uint32_t width, uint32_t height,
  * out_ms[2] = blur+max, out_ms[3] = row pass closing a cycle (norm+error). */
 int imresh_profile_kernels(uint32_t width, uint32_t height, uint32_t reps, float *out_ms,
                            void *stream);
+/* Same for the two kernels of imresh_diffraction_intensity_device:
+ * out_ms[0] = column pass (real plane -> forward columns), out_ms[1] = row
+ * pass (forward rows -> |z|). */
+int imresh_profile_forward_model(uint32_t width, uint32_t height, uint32_t reps, float *out_ms,
+                                 void *stream);
 
 #ifdef __cplusplus
 }
