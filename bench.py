@@ -324,8 +324,9 @@ def forward_model_bench(L, obj: np.ndarray, reps: int = 50) -> dict:
     at the object's size: kernel-exact durations of its two fused passes
     (real-input column pass 12 B/element, modulus-output row pass 12 B/element),
     wall-clock rate of back-to-back calls (each preceded by a 16 MiB device
-    copy restoring the object, timed with it) and parity against numpy in
-    float64."""
+    copy restoring the object, timed with it), parity against numpy in
+    float64, and the Poisson measurement simulation on its output
+    (imresh_poisson_modulus, wall clock per call)."""
     h, w = obj.shape
     x = np.ascontiguousarray(obj, dtype=np.float32)
     d_src = im.DeviceBuffer.from_array(x)
@@ -345,6 +346,14 @@ def forward_model_bench(L, obj: np.ndarray, reps: int = 50) -> dict:
         step()
     im.sync()
     dt = (time.perf_counter() - t0) / reps
+    # measurement simulation on the forward model's output (in place, repeated)
+    im.check(L.imresh_poisson_modulus(d_io.ptr, w, h, 1e9, 1, None), "poisson")
+    im.sync()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        im.check(L.imresh_poisson_modulus(d_io.ptr, w, h, 1e9, r, None), "poisson")
+    im.sync()
+    dt_pois = (time.perf_counter() - t0) / reps
     ms = (im.ctypes.c_float * 2)()
     im.check(L.imresh_profile_forward_model(w, h, 20, ms, None), "profile forward model")
     n = w * h
@@ -354,7 +363,9 @@ def forward_model_bench(L, obj: np.ndarray, reps: int = 50) -> dict:
             "kernels": {"col_real_fwd": {"ms": round(col_ms, 5), "GB/s": round(12 * n / (col_ms * 1e-3) / 1e9, 1)},
                         "row_fwd_norm": {"ms": round(row_ms, 5), "GB/s": round(12 * n / (row_ms * 1e-3) / 1e9, 1)}},
             "alg_GBps_24B": round(24 * n / ((col_ms + row_ms) * 1e-3) / 1e9, 1),
-            "rel_l2_vs_numpy_fp64": rel}
+            "rel_l2_vs_numpy_fp64": rel,
+            "poisson_modulus_ms": round(dt_pois * 1e3, 4),
+            "poisson_modulus_note": "sqrt(Poisson(1e9 |F|^2 / sum |F|^2)) per element, wall clock per call"}
 
 
 def main() -> int:

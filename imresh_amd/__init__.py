@@ -126,6 +126,7 @@ _SIGS = {
     "imresh_fft_supported": (c_int, [c_uint32]),
     "imresh_diffraction_intensity": (c_int, [_P, c_uint32, c_uint32, _P, c_int]),
     "imresh_diffraction_intensity_device": (c_int, [_P, c_uint32, c_uint32, _P]),
+    "imresh_poisson_modulus": (c_int, [_P, c_uint32, c_uint32, c_double, c_uint64, _P]),
     "imresh_fft_shift": (c_int, [_P, c_uint32, c_uint32, c_uint32, _P]),
     "imresh_autotune": (c_int, [c_uint32, c_uint32, _P, POINTER(c_int), POINTER(c_int)]),
     "imresh_variant_counts": (c_int, [c_uint32, c_uint32, POINTER(c_int), POINTER(c_int)]),

@@ -93,6 +93,10 @@ struct BlurArgs {
 constexpr int kScalStride = 4;
 constexpr int kFlagStride = 2;
 constexpr int kBlurWeightRow = 64;
+// double scratch of a workspace: per-workgroup sums + total of the Poisson
+// measurement simulation
+constexpr int kPoissonSumBlocks = 256;
+constexpr int kAuxDoubles = kPoissonSumBlocks + 1;
 
 // A compile-time plan (plans.hpp) of one line length with its twiddles.
 // A line length is transformed either by its own compiled plan or, when it
@@ -170,6 +174,9 @@ int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float t
                         float *res, int *done, unsigned *cycles, hipStream_t s);
 int launch_copy_from_real(float *out, const float2 *in, size_t n, hipStream_t s);
 int launch_random_phase(float2 *X, const float *M, size_t n, hipStream_t s);
+// measurement simulation: modulus |F| -> sqrt(Poisson(photons |F|^2 / sum |F|^2)),
+// aux = kAuxDoubles doubles of device scratch
+int launch_poisson_modulus(float *m, size_t n, double photons, uint64_t seed, double *aux, hipStream_t s);
 int launch_fft_shift(void *d, unsigned nx, unsigned ny, unsigned elem_bytes, hipStream_t s);
 int launch_gather_images(float *dst, const float *const *src, size_t n, int batch, hipStream_t s);
 int launch_scatter_real(float *const *dst, const float2 *X, size_t n, int batch, hipStream_t s);

@@ -267,6 +267,17 @@ int imresh_fft2d(float *out_dev, const float *in_dev, uint32_t width, uint32_t h
  * the _device variant takes device memory and is asynchronous. */
 int imresh_diffraction_intensity(float *io, uint32_t width, uint32_t height, void *stream, int async);
 int imresh_diffraction_intensity_device(float *io_dev, uint32_t width, uint32_t height, void *stream);
+/* Measurement simulation (no reference counterpart; the reference reads
+ * recorded or host-computed intensities): a device modulus |F| (e.g. from
+ * imresh_diffraction_intensity_device) is replaced in place by
+ * sqrt(k), k ~ Poisson(photons |F|^2 / sum |F|^2) per element, i.e. the
+ * modulus of a measurement that counted `photons` photons in expectation
+ * (0 < photons <= 1e30).
+ * Deterministic for a given seed (counter-based Philox4x32-10 stream per
+ * element; the sum in a fixed order), independent of the launch shape.
+ * Asynchronous. */
+int imresh_poisson_modulus(float *modulus_dev, uint32_t width, uint32_t height, double photons,
+                           uint64_t seed, void *stream);
 /* 1 if an FFT plan exists for this line length, else 0. */
 int imresh_fft_supported(uint32_t n);
 /* fftShift (src/imresh/libs/fftShift.cpp:42-61) in place on a device

@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_double, c_float, c_int, c_uint, c_void_p
+from ctypes import POINTER, c_double, c_float, c_int, c_uint, c_uint64, c_void_p
 
 import numpy as np
 
@@ -65,6 +65,19 @@ def _declare(lib):
             f = getattr(lib, f"{name}_{s}")
             f.restype = res
             f.argtypes = args
+    P = c_void_p
+    for name, res, args in (("orc_philox4x32_10", None, [P, P, P]),
+                            ("orc_exp_d", c_double, [c_double]),
+                            ("orc_log_d", c_double, [c_double]),
+                            ("orc_sqrt_d", c_double, [c_double]),
+                            ("orc_log_factorial", c_double, [c_double]),
+                            ("orc_poisson", c_double, [c_double, c_uint64, c_uint64]),
+                            ("orc_sq_sum_fixed", c_double, [P, ctypes.c_size_t]),
+                            ("orc_poisson_modulus", None, [P, ctypes.c_size_t, c_double, c_uint64]),
+                            ("orc_poisson_many", None, [P, ctypes.c_size_t, c_uint64, c_uint64, P])):
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
     lib.orc_fft_shift.restype = None
     lib.orc_fft_shift.argtypes = [c_void_p, c_uint, c_uint, c_uint]
     lib.orc_sigma_schedule.restype = None
@@ -306,6 +319,52 @@ def fft_shift(a):
     h, w = x.shape
     lib().orc_fft_shift(x.ctypes.data_as(c_void_p), w, h, x.itemsize)
     return x
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 of a 4-word counter and a 2-word key."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def exp_d(x):
+    return lib().orc_exp_d(float(x))
+
+
+def log_d(x):
+    return lib().orc_log_d(float(x))
+
+
+def sqrt_d(x):
+    return lib().orc_sqrt_d(float(x))
+
+
+def log_factorial(k):
+    return lib().orc_log_factorial(float(k))
+
+
+def poisson_many(lam, seed, idx0=0):
+    """Poisson samples of the counter-based stream: element idx0 + j draws
+    with rate lam[j]."""
+    l = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.empty_like(l)
+    lib().orc_poisson_many(l.ctypes.data, l.size, seed, idx0, out.ctypes.data)
+    return out
+
+
+def sq_sum_fixed(m):
+    a = np.ascontiguousarray(m, dtype=np.float32)
+    return lib().orc_sq_sum_fixed(a.ctypes.data, a.size)
+
+
+def poisson_modulus(m, photons, seed):
+    """The measurement simulation: m <- sqrt(Poisson(photons m^2 / sum m^2)) (a copy)."""
+    a = np.array(m, dtype=np.float32, copy=True, order="C")
+    lib().orc_poisson_modulus(a.ctypes.data, a.size, float(photons), seed)
+    return a
 
 
 def sigma_schedule(sigma0, sigma_change, n):

@@ -30,6 +30,9 @@
 #ifndef IMRESH_ORACLE_H
 #define IMRESH_ORACLE_H
 
+#include <stddef.h>
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -78,6 +81,20 @@ typedef struct orc_sw_stats {
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)
+
+/* measurement simulation (the specification of the device sampler):
+ * Philox4x32-10, exp/log/sqrt/log k! from IEEE operations, Poisson(lambda)
+ * per element from the counter-based stream (seed, index), the fixed-order
+ * sum of squares and the whole modulus transformation */
+void   orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_exp_d(double x);
+double orc_log_d(double x);
+double orc_sqrt_d(double x);
+double orc_log_factorial(double k);
+double orc_poisson(double lam, uint64_t seed, uint64_t idx);
+double orc_sq_sum_fixed(const float *m, size_t n);
+void   orc_poisson_modulus(float *m, size_t n, double photons, uint64_t seed);
+void   orc_poisson_many(const double *lam, size_t n, uint64_t seed, uint64_t idx0, double *out);
 
 /* fftShift (fftShift.cpp:42-61) on elements of elem_bytes (4 or 8) */
 void orc_fft_shift(void *data, unsigned nx, unsigned ny, unsigned elem_bytes);

@@ -479,3 +479,90 @@ def test_diffraction_intensity_closed_forms():
     F = oracle.diffraction_intensity(x, dtype=np.float64)
     assert abs((F ** 2).sum() - h * w * (x ** 2).sum()) < 1e-9 * h * w * (x ** 2).sum()
     assert np.allclose(F, np.roll(F[::-1, ::-1], (1, 1), axis=(0, 1)), rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- measurement simulation
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors published with Random123
+    (kat_vectors: zero, all-ones and pi-digit counters/keys)."""
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+           ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+           ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+            [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+    for ctr, key, exp in kat:
+        assert list(oracle.philox4x32_10(ctr, key)) == exp
+
+
+def test_ieee_elementary_functions_vs_libm():
+    """The sampler's exp / log / sqrt / log k! (IEEE operations only) agree
+    with libm within 2 ulp (exp, log, sqrt) and 1e-14 relative (log k!)."""
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([rng.uniform(-40, 0, 2000), rng.uniform(-744, 700, 2000), [0.0, -1e-300, 1e-12]])
+    for x in xs:
+        e = math.exp(x)
+        assert abs(oracle.exp_d(x) - e) <= 2 * math.ulp(e), x
+    ys = np.concatenate([10.0 ** rng.uniform(-300, 300, 3000), rng.uniform(0.5, 2.0, 2000), [1.0, 2.0, 0.5]])
+    for y in ys:
+        l = math.log(y)
+        assert abs(oracle.log_d(y) - l) <= 2 * math.ulp(l) + 1e-300, y
+        if 1e-30 <= y <= 1e30:   # the sampler's domain (photons <= 1e30)
+            s = math.sqrt(y)
+            assert abs(oracle.sqrt_d(y) - s) <= math.ulp(s), y
+    assert oracle.log_d(0.0) == -math.inf and oracle.exp_d(-800.0) == 0.0
+    for k in list(range(0, 40)) + [50, 99, 1000, 12345, 10 ** 6, 3 * 10 ** 9]:
+        lf = math.lgamma(k + 1)
+        assert abs(oracle.log_factorial(k) - lf) <= 1e-14 * max(1.0, lf), k
+
+
+@pytest.mark.parametrize("lam", [0.05, 0.7, 3.0, 9.99, 10.0, 37.5, 1000.0, 2.5e6])
+def test_poisson_sampler_distribution(lam):
+    """Inversion (lambda < 10) and PTRS (lambda >= 10) draw Poisson(lambda):
+    mean and variance within 5 sigma, chi-square goodness of fit against the
+    exact pmf (bins with >= 20 expected counts), and integer outputs."""
+    from scipy import stats
+    n = 200000
+    k = oracle.poisson_many(np.full(n, lam), seed=2024 + int(lam * 10), idx0=7)
+    assert np.all(k == np.floor(k)) and k.min() >= 0
+    m, v = k.mean(), k.var()
+    assert abs(m - lam) < 5 * math.sqrt(lam / n)
+    # variance of the sample variance of Poisson: (lam + 2 lam^2 (n/(n-1))) / n ~ (lam + 2 lam^2) / n
+    assert abs(v - lam) < 5 * math.sqrt((lam + 2 * lam * lam) / n)
+    lo, hi = int(stats.poisson.ppf(1e-7, lam)), int(stats.poisson.ppf(1 - 1e-7, lam)) + 1
+    edges = np.arange(lo, hi + 1)
+    pmf = stats.poisson.pmf(edges, lam)
+    # merge into bins of >= 20 expected counts
+    bins, cur, acc = [], [], 0.0
+    for e, p in zip(edges, pmf):
+        cur.append(e)
+        acc += p
+        if acc * n >= 20:
+            bins.append((cur, acc))
+            cur, acc = [], 0.0
+    if cur:
+        bins[-1] = (bins[-1][0] + cur, bins[-1][1] + acc)
+    obs = np.array([np.isin(k, b).sum() if len(b) < 64 else ((k >= b[0]) & (k <= b[-1])).sum()
+                    for b, _ in bins], dtype=np.float64)
+    exp = np.array([p for _, p in bins]) * n
+    exp *= obs.sum() / exp.sum()
+    chi2 = ((obs - exp) ** 2 / exp).sum()
+    assert stats.chi2.sf(chi2, len(bins) - 1) > 1e-6, (chi2, len(bins))
+
+
+def test_poisson_modulus_sum_order_and_photon_budget():
+    """The fixed-order sum agrees with an exactly rounded sum to 1e-13; the
+    simulated measurement is sqrt of integer counts whose total is
+    Poisson(photons); zero and constant moduli behave."""
+    rng = np.random.default_rng(3)
+    m = (rng.random(300001) * 100).astype(np.float32)
+    exact = math.fsum(float(x) * float(x) for x in m.astype(np.float64))
+    assert abs(oracle.sq_sum_fixed(m) - exact) <= 1e-13 * exact
+    out = oracle.poisson_modulus(m, 1e8, seed=5)
+    k = out.astype(np.float64) ** 2
+    assert np.allclose(k, np.round(k), rtol=0, atol=1e-3 * np.maximum(1, k))
+    assert abs(np.round(k).sum() - 1e8) < 6 * math.sqrt(1e8)
+    assert np.array_equal(oracle.poisson_modulus(np.zeros(100, np.float32), 1e6, 1), np.zeros(100, np.float32))
+    c = oracle.poisson_modulus(np.ones(10000, np.float32), 5e4, 9)   # lambda = 5 everywhere
+    assert abs((c.astype(np.float64) ** 2).mean() - 5.0) < 5 * math.sqrt(5.0 / 10000)
+    # the seed selects the stream; the same seed repeats it
+    assert np.array_equal(oracle.poisson_modulus(m[:1000], 1e6, 5), oracle.poisson_modulus(m[:1000], 1e6, 5))
+    assert not np.array_equal(oracle.poisson_modulus(m[:1000], 1e6, 5), oracle.poisson_modulus(m[:1000], 1e6, 6))
