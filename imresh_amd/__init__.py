@@ -135,6 +135,10 @@ _SIGS = {
     "imresh_task_queue_deinit": (c_int, []),
     "imresh_add_task": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), TASK_CALLBACK, _P]),
     "imresh_profile_kernels": (c_int, [c_uint32, c_uint32, c_uint32, POINTER(c_float), _P]),
+    "imresh_read_png": (c_int, [ctypes.c_char_p, _P, ctypes.c_size_t, POINTER(c_uint32), POINTER(c_uint32)]),
+    "imresh_read_tiff": (c_int, [ctypes.c_char_p, _P, ctypes.c_size_t, POINTER(c_uint32), POINTER(c_uint32)]),
+    "imresh_read_txt": (c_int, [ctypes.c_char_p, _P, ctypes.c_size_t, POINTER(c_uint32), POINTER(c_uint32)]),
+    "imresh_write_png": (c_int, [ctypes.c_char_p, _P, c_uint32, c_uint32]),
     "imresh_profile_forward_model": (c_int, [c_uint32, c_uint32, c_uint32, POINTER(c_float), _P]),
 }
 
@@ -323,6 +327,30 @@ def hybrid_input_output(io: np.ndarray, mask: np.ndarray, n_cycles: int = 0,
                                             target_error, hio_beta, stream, ctypes.byref(r)),
           "imresh_hybrid_input_output")
     return r
+
+
+def read_image(path: str) -> np.ndarray:
+    """readPNG / readTiff / readTxt by file extension (.png, .tif/.tiff,
+    anything else as text): a float32 H x W array."""
+    ext = os.path.splitext(path)[1].lower()
+    fn = {".png": "imresh_read_png", ".tif": "imresh_read_tiff", ".tiff": "imresh_read_tiff"}.get(
+        ext, "imresh_read_txt")
+    L = _lib()
+    w, h = c_uint32(), c_uint32()
+    p = os.fsencode(path)
+    check(getattr(L, fn)(p, None, 0, ctypes.byref(w), ctypes.byref(h)), fn)
+    out = np.empty((h.value, w.value), np.float32)
+    check(getattr(L, fn)(p, out.ctypes.data, out.size, ctypes.byref(w), ctypes.byref(h)), fn)
+    return out
+
+
+def write_png(path: str, data: np.ndarray) -> None:
+    """writeOutPNG: 16-bit RGB scaled by the maximum."""
+    a = np.ascontiguousarray(data, dtype=np.float32)
+    if a.ndim != 2:
+        raise ValueError("H x W array expected")
+    check(_lib().imresh_write_png(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0]),
+          "imresh_write_png")
 
 
 def diffraction_intensity(io: np.ndarray, stream=None) -> np.ndarray:

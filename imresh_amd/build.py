@@ -26,7 +26,7 @@ NUM_PLAN_GROUPS = 7   # IMR_NUM_PLAN_GROUPS in csrc/plans.hpp
 UNITS = ([("k_row.hip", f"k_row_g{g}", [f"-DIMR_GROUP={g}"]) for g in range(NUM_PLAN_GROUPS)]
          + [("k_col.hip", f"k_col_g{g}", [f"-DIMR_GROUP={g}"]) for g in range(NUM_PLAN_GROUPS)]
          + [(s, s.rsplit(".", 1)[0], []) for s in ("k_misc.hip", "k_poisson.hip", "plan.cpp", "engine.cpp",
-                                                    "batch.cpp", "serve.cpp", "task_queue.cpp")])
+                                                    "batch.cpp", "serve.cpp", "task_queue.cpp", "io.cpp")])
 
 
 def hipcc() -> str:
@@ -70,12 +70,29 @@ def build(verbose: bool = False) -> str:
         list(ex.map(lambda u: _compile(u[0][0], u[1], u[0][2], verbose), zip(UNITS, objs)))
     out = os.path.join(LIBDIR, LIBNAME)
     tmp = out + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"]
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, out)
+    _build_example(verbose)
     return out
+
+
+EXAMPLE = os.path.join(LIBDIR, "imresh_queue_example")
+
+
+def _build_example(verbose: bool) -> None:
+    """The C++ drop-in example (examples/queue_png.cpp) against the built library."""
+    src = os.path.join(ROOT, "imresh_amd", "examples", "queue_png.cpp")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", EXAMPLE + ".tmp",
+           "-L", LIBDIR, "-limresh_hip", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("example build failed:\n" + r.stdout + r.stderr)
+    os.replace(EXAMPLE + ".tmp", EXAMPLE)
 
 
 def build_oracle(verbose: bool = False) -> None:

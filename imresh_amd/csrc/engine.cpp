@@ -676,6 +676,7 @@ const char *imresh_strerror(int status)
     case IMRESH_ERR_KERNEL_TOO_LARGE: return "Gaussian kernel wider than 63 taps";
     case IMRESH_ERR_NOT_INITIALIZED: return "task queue not initialized";
     case IMRESH_ERR_BUDGET: return "request exceeds the HBM budget";
+    case IMRESH_ERR_IO: return "file unreadable, unwritable or of an unsupported format";
     default: return "unknown status";
     }
 }

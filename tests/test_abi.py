@@ -47,7 +47,13 @@ def test_exports_cpp_dropin_api():
                 "imresh::libs::hybridInputOutput(float*, unsigned char const*, std::vector<unsigned int",
                 "imresh::libs::diffractionIntensity(float*, unsigned int, unsigned int, ihipStream_t*, bool)",
                 "imresh::libs::fftShift<float>(float*, unsigned int, unsigned int)",
-                "imresh::libs::fftShift<float [2]>(float (*) [2], unsigned int, unsigned int)"):
+                "imresh::libs::fftShift<float [2]>(float (*) [2], unsigned int, unsigned int)",
+                "imresh::io::readInFuncs::readPNG(",
+                "imresh::io::readInFuncs::readTiff(",
+                "imresh::io::readInFuncs::readTxt(",
+                "imresh::io::writeOutFuncs::writeOutPNG(float*, unsigned int, unsigned int, std::",
+                "imresh::io::writeOutFuncs::writeOutAndFreePNG(",
+                "imresh::io::writeOutFuncs::justFree("):
         assert sym in out, sym
 
 
