@@ -11,12 +11,12 @@
 //         channel as a 16-bit value (8-bit samples times 256, as pngwriter's
 //         read(); gray counts as red, palettes are looked up) over 65535.
 //         Writing produces 16-bit RGB with R = G = B = int(v/max*65535 + 0.5f)
-//         clamped to [0, 65535] (writeOutFuncs.cpp:69-118).
+//         clamped to [0, 65535] (writeOutFuncs.cpp:69-114).
 //   TIFF  baseline TIFF 6.0, one 32-bit IEEE float sample per pixel,
 //         uncompressed strips, either byte order (the reference reads this
 //         layout through libtiff, ReadTiff.cpp).
 //   text  whitespace-separated floats, one image row per line (readTxt,
-//         readInFuncs.cpp:55-104).
+//         readInFuncs.cpp:56-104).
 // HDF5 (readHDF5 / writeOutHDF5 through libSplash) has no counterpart: the
 // HDF5 and libSplash libraries are not available to this build.
 #include <zlib.h>

@@ -303,17 +303,17 @@ int imresh_set_variants(uint32_t width, uint32_t height, int row_variant, int co
 /* Readers: with out == NULL only *width / *height are set (size query);
  * otherwise out receives width*height floats (capacity in floats, too small:
  * IMRESH_ERR_INVALID).  IMRESH_ERR_IO: unreadable file or unsupported layout.
- * PNG (readPNG, readInFuncs.cpp:109-162): red channel as 16 bits (8-bit
+ * PNG (readPNG, readInFuncs.cpp:108-157): red channel as 16 bits (8-bit
  * samples x 256) / 65535, buffer row iy = file row height-1-iy (pngwriter's
  * bottom-left origin); 1/2/4/8/16-bit gray, palette, RGB(A), gray+alpha,
- * non-interlaced.  TIFF (readTiff, :165-200 / ReadTiff.cpp): 32-bit IEEE
+ * non-interlaced.  TIFF (readTiff, :160-195 / ReadTiff.cpp): 32-bit IEEE
  * float, one sample per pixel, uncompressed strips, either byte order.
- * Text (readTxt, :55-104): whitespace-separated floats, width = values on
+ * Text (readTxt, :56-104): whitespace-separated floats, width = values on
  * the first line, height = count / width. */
 int imresh_read_png(const char *path, float *out, size_t capacity, uint32_t *width, uint32_t *height);
 int imresh_read_tiff(const char *path, float *out, size_t capacity, uint32_t *width, uint32_t *height);
 int imresh_read_txt(const char *path, float *out, size_t capacity, uint32_t *width, uint32_t *height);
-/* writeOutPNG (writeOutFuncs.cpp:69-118): 16-bit RGB, R = G = B =
+/* writeOutPNG (writeOutFuncs.cpp:69-114): 16-bit RGB, R = G = B =
  * int(v/max*65535 + 0.5f) clamped to [0, 65535] (max 0 -> 1), buffer row iy
  * = file row height-1-iy. */
 int imresh_write_png(const char *path, const float *data, uint32_t width, uint32_t height);

@@ -1,7 +1,7 @@
 """File formats of the task queue's callbacks and readers (host code, no GPU):
-readPNG / writeOutPNG (readInFuncs.cpp:109-162, writeOutFuncs.cpp:69-118),
-readTiff (readInFuncs.cpp:165-200, ReadTiff.cpp) and readTxt
-(readInFuncs.cpp:55-104) through the C ABI.
+readPNG / writeOutPNG (readInFuncs.cpp:108-157, writeOutFuncs.cpp:69-114),
+readTiff (readInFuncs.cpp:160-195, ReadTiff.cpp) and readTxt
+(readInFuncs.cpp:56-104) through the C ABI.
 
 Pins: the reference's own round-trip test (testReadWrite.cpp:55-185: random
 data from srand(876512789), log-spaced square sizes, repeated write/read
