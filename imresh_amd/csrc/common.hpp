@@ -38,21 +38,27 @@ inline int from_hip(hipError_t e)
 }  // namespace imr
 
 namespace imr {
-// Kernel-exact timing for the profiler entry point: while `start` is set on
-// the calling thread, IMR_LAUNCH records the two events around the kernel's
-// own execution (hipExtLaunchKernelGGL), which is what rocprofv3's kernel
-// trace measures; otherwise it is a plain launch.
+// Kernel-exact timing for the profiler entry point: while `stop` is set on
+// the calling thread, IMR_LAUNCH records `start` when the first kernel of the
+// timed operation begins and `stop` when each kernel ends
+// (hipExtLaunchKernelGGL), so the pair brackets exactly the kernels'
+// execution, as rocprofv3's kernel trace measures it, also for an operation
+// of several launches; otherwise it is a plain launch.
 struct ProfEvents {
     hipEvent_t start = nullptr, stop = nullptr;
+    bool started = false;
 };
 extern thread_local ProfEvents g_prof;
 }  // namespace imr
 
-#define IMR_LAUNCH(K, grid, block, lds, s, ...)                                                \
-    do {                                                                                       \
-        if (imr::g_prof.start != nullptr)                                                      \
-            hipExtLaunchKernelGGL(K, grid, block, lds, s, imr::g_prof.start, imr::g_prof.stop, \
-                                  0, __VA_ARGS__);                                             \
-        else                                                                                   \
-            hipLaunchKernelGGL(K, grid, block, lds, s, __VA_ARGS__);                           \
+#define IMR_LAUNCH(K, grid, block, lds, s, ...)                                                 \
+    do {                                                                                        \
+        if (imr::g_prof.stop != nullptr) {                                                      \
+            hipExtLaunchKernelGGL(K, grid, block, lds, s,                                       \
+                                  imr::g_prof.started ? nullptr : imr::g_prof.start,            \
+                                  imr::g_prof.stop, 0, __VA_ARGS__);                            \
+            imr::g_prof.started = true;                                                         \
+        } else {                                                                                \
+            hipLaunchKernelGGL(K, grid, block, lds, s, __VA_ARGS__);                            \
+        }                                                                                       \
     } while (0)

@@ -69,7 +69,28 @@ struct ColArgs {
     size_t istride;        // elements between consecutive images
     int len;               // column length H (Bluestein kernels)
     const float2 *bs;      // Bluestein tables of H (set by launch_col)
+    // sub-blocks (the middle step of a four-step column transform): grid.y
+    // = images x sub; block s of image b starts at b * istride + s *
+    // sub_xstride in X and at b * istride + s * sub_mstride in M, whose rows
+    // are mpitch elements apart.  sub 0 / 1 and mpitch 0: one block, M rows W
+    // apart.
+    int sub;
+    size_t sub_xstride, sub_mstride;
+    int mpitch;
 };
+
+// the radix-8 outer steps of a four-step column transform of length H =
+// 8 * H2 (k_step8.hip): X of an image viewed as [8][H2][W]
+struct Step8Args {
+    float2 *X;
+    const int *done;      // early-exit flags of the images (NULL: never)
+    const float2 *tw;     // exp(-2 pi i j / H), j < H
+    int W, H2;
+    size_t istride;       // elements between consecutive images
+};
+// forward step (DIR -1: 8-point DFTs over rows n2 + H2*n1, then twiddles
+// w^(n2*k1)) or inverse step (DIR +1: conjugate twiddles, then 8-point DFTs)
+int launch_step8(int dir, const Step8Args &a, int batch, hipStream_t s);
 
 struct BlurArgs {
     const float *in;
@@ -112,7 +133,13 @@ struct LinePlan {
     int blue;        // 1: Bluestein
     int m;           // convolution length (Bluestein)
     const float2 *bs;   // Bluestein tables (device)
+    // four-step column pass (n = 8 * fs_h2, see launch_col): 0 = off
+    int fs_h2;
+    const float2 *fs_tw;   // exp(-2 pi i j / n), j < n (device)
 };
+// column lengths at or above this use the four-step column pass for the
+// fused projection: their own tile holds only 2 columns (16-byte rows)
+constexpr int kFourStepMinLine = 8192;
 
 // variant index of the Bluestein kernels: kBlueVariant + {0: the plan's
 // column tile, 1: half as many columns per workgroup, 2 / 3: the same with

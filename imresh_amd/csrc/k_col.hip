@@ -36,7 +36,10 @@ enum ColKernel { CK_FWD, CK_INV, CK_PROJ };
 template <class P, int C, bool SPLIT, int K, bool PF = false, bool BL = false, int STG = 0>
 __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 {
-    const int b = blockIdx.y;   // image of a batch
+    // image b of a batch and, for the middle step of a four-step column
+    // pass, its sub-block sb (grid.y = images x sub)
+    const int sub = a.sub > 1 ? a.sub : 1;
+    const int b = (int) blockIdx.y / sub, sb = (int) blockIdx.y - b * sub;
     if (a.done != nullptr && a.done[b * kFlagStride]) return;
     if constexpr (STG > 0) {
         // phase offset: the second half of the grid shares CUs with the first
@@ -59,7 +62,9 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
     const int W = a.W;
     // workgroup base (uniform) + 32-bit per-lane offsets e*W + c; columns past
     // the edge read column 0 of the tile and never store
-    const size_t wb = (size_t) b * a.istride + (size_t) bid * C;
+    const size_t wb = (size_t) b * a.istride + (size_t) sb * a.sub_xstride + (size_t) bid * C;
+    const size_t mb = (size_t) b * a.istride + (size_t) sb * a.sub_mstride + (size_t) bid * C;
+    const int MP = a.mpitch > 0 ? a.mpitch : W;   // row pitch of M (CK_PROJ)
     const int co = valid ? c : 0;
     using Lds = std::conditional_t<SPLIT, LdsS<C>, LdsC<C>>;
     const Lds L{(decltype(Lds::p)) smem, c};
@@ -85,15 +90,15 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
         xfft<P, +1, BL>(v, tw, t, L, bl, g0, sl);
     } else if constexpr (K == CK_PROJ) {
         cf *X = (cf *) a.X + wb;
-        const float *M = a.M + wb;
+        const float *M = a.M + mb;
         auto g0 = [=](int e) { return at32(X, e * W + co); };
         auto pre = [=](int e) {
-            if constexpr (PF) return at32(M, e * W + co);
+            if constexpr (PF) return at32(M, e * MP + co);
             else return NoPre{};
         };
         auto mid = [=](int e, cf z, auto m) {
             if constexpr (PF) return modulus_project_fused(z, m);
-            else return modulus_project_fused(z, at32(M, e * W + co));
+            else return modulus_project_fused(z, at32(M, e * MP + co));
         };
         auto sl = [=](int e, cf x) { if (valid) at32(X, e * W + co) = x; };
         xpair<P, -1, +1, BL>(v, tw, t, L, bl, g0, pre, mid, sl);
@@ -109,7 +114,8 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
 {
     static_assert(C * P::T <= 1024, "column workgroup too large");
     static_assert(col_lds<P, C, SPLIT>() <= kLdsBudget, "column tile exceeds the LDS budget");
-    const dim3 grid((a.W + C - 1) / C, a.batch > 1 ? a.batch : 1), block(C * P::T);
+    const int images = a.batch > 1 ? a.batch : 1;
+    const dim3 grid((a.W + C - 1) / C, images * (a.sub > 1 ? a.sub : 1)), block(C * P::T);
     constexpr size_t lds = col_lds<P, C, SPLIT>();
     if constexpr (lds > 64 * 1024) {
         static const hipError_t attr = [&] {

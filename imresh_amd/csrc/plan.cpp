@@ -205,6 +205,22 @@ int get_line_plan(int device, int n, LinePlan *out)
     lp.L = pi->L;
     lp.C = pi->C;
     lp.tw = dev;
+    if (n >= kFourStepMinLine && n % 8 == 0 && find_info(n / 8) != nullptr) {
+        // four-step column pass: the twiddles exp(-2 pi i j / n) of its outer steps
+        std::vector<float2> t(n);
+        for (int j = 0; j < n; ++j) {
+            const double ang = -2.0 * M_PI * (double) j / (double) n;
+            t[j] = make_float2((float) std::cos(ang), (float) std::sin(ang));
+        }
+        float2 *fs = nullptr;
+        hipSetDevice(device);
+        e = hipMalloc(&fs, sizeof(float2) * t.size());
+        if (e == hipSuccess) e = hipMemcpy(fs, t.data(), sizeof(float2) * t.size(), hipMemcpyHostToDevice);
+        hipSetDevice(prev);
+        if (e != hipSuccess) return from_hip(e);
+        lp.fs_h2 = n / 8;
+        lp.fs_tw = fs;
+    }
     g_lines[{device, n}] = lp;
     *out = lp;
     return IMRESH_OK;
@@ -280,8 +296,49 @@ int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
     }
 }
 
+namespace {
+
+// The fused column pass of a four-step length H = 8 * H2: the forward outer
+// step, the existing fused forward/projection/inverse kernel of length H2
+// on the 8 blocks of H2 rows of every image (block k1 holds frequencies
+// k1 + 8 k2, so its modulus rows are k1 + 8 k2: M offset k1 rows, pitch 8
+// rows), the inverse outer step.  52 bytes per element instead of 20, but
+// every access covers whole 64-column row segments.
+int launch_col_fourstep(const Plan2D &p, ColArgs a, hipStream_t s)
+{
+    const int h2 = p.col.fs_h2;
+    const int images = a.batch > 1 ? a.batch : 1;
+    const size_t istride = images > 1 ? a.istride : (size_t) p.W * p.H;
+    Step8Args sa{};
+    sa.X = a.X;
+    sa.done = a.done;
+    sa.tw = p.col.fs_tw;
+    sa.W = p.W;
+    sa.H2 = h2;
+    sa.istride = istride;
+    IMR_CHECK(launch_step8(-1, sa, images, s));
+    int dev = 0;
+    IMR_HIP_CHECK(hipGetDevice(&dev));
+    Plan2D q = p;
+    q.H = h2;
+    IMR_CHECK(get_line_plan(dev, h2, &q.col));
+    q.colVar = col_variants(q.col) / 2;   // the prefetching default tile
+    ColArgs b = a;
+    b.batch = images;
+    b.sub = 8;
+    b.istride = istride;
+    b.sub_xstride = (size_t) h2 * p.W;
+    b.sub_mstride = (size_t) p.W;
+    b.mpitch = 8 * p.W;
+    IMR_CHECK(launch_col(COL_PROJ, q, b, s));
+    return launch_step8(+1, sa, images, s);
+}
+
+}  // namespace
+
 int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
 {
+    if (mode == COL_PROJ && p.col.fs_h2 > 0 && a.sub <= 1) return launch_col_fourstep(p, a, s);
     a.W = p.W;
     a.len = p.H;
     a.bs = p.col.bs;

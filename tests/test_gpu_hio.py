@@ -46,7 +46,7 @@ def _gpu_iterations(M, g, gp, mask, beta, n_iter):
     return dg.download(np.complex64, (h, w)), dgp.download(np.complex64, (h, w))
 
 
-@pytest.mark.parametrize("w,h", [(2048, 2048), (1024, 1024), (4096, 256), (256, 2048)])
+@pytest.mark.parametrize("w,h", [(2048, 2048), (1024, 1024), (4096, 256), (256, 2048), (72, 8192)])
 def test_every_kernel_variant_computes_the_same_bits(w, h):
     """The autotuner picks among row/column kernel variants that differ only
     in tiling and scheduling (threads per line, lines per workgroup,
@@ -239,9 +239,26 @@ def test_config_2048_full_run_deterministic():
     print(f"2048^2 20x20: final error {res.last_error:.6g}, NCC to object {syn.ncc(obj, outs[0]):.4f}")
 
 
+@pytest.mark.parametrize("w", [200, 8])
+def test_one_hio_iteration_8192_high(w):
+    """8192-point columns take the four-step column pass (radix-8 outer
+    steps around the fused 1024-point kernel on 8 row blocks, modulus rows
+    k1 + 8 k2): one iteration against the float64 oracle, and the gPrev
+    constraint exact."""
+    obj, M, g, gp, mask = _state(w, 8192, index=w, shapes=8)
+    g_gpu, gp_gpu = _gpu_iterations(M, g, gp, mask, 0.9, 1)
+    g32, gp32 = oracle.hio_iterations(g, gp, mask, M, 0.9, 1, np.float32)
+    g64, _ = oracle.hio_iterations(g.astype(np.complex128), gp.astype(np.complex128),
+                                   mask.astype(np.float64), M.astype(np.float64), 0.9, 1, np.float64)
+    e64 = syn.rel_l2(g_gpu, g64)
+    print(f"{w}x8192: one iteration rel L2 vs fp64 oracle {e64:.3e}")
+    assert e64 < 1e-5
+    assert np.array_equal(gp_gpu.view(np.float32), gp32.view(np.float32))
+
+
 def test_one_hio_iteration_8192():
     """The large-image configuration's shape (8192^2: 8192-point rows in one
-    512-thread workgroup, 2-column tiles of 8192-point columns)."""
+    512-thread workgroup, four-step 8192-point columns)."""
     oracle.set_fft_threads(16)
     obj, M, g, gp, mask = _state(8192, 8192, index=8192, shapes=64)
     g_gpu, gp_gpu = _gpu_iterations(M, g, gp, mask, 0.9, 1)
