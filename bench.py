@@ -46,7 +46,7 @@ N_CYCLES, N_HIO = 20, 20
 HBM_PEAK_GBS = 8000.0
 FLT_MIN = float(np.finfo(np.float32).tiny)
 
-# compulsory HBM bytes per element of each fused kernel (see DESIGN.md):
+# compulsory HBM bytes per element of each fused kernel:
 #   row pass  (inverse rows, constraint, forward rows): read X 8 + G 8 + mask 1, write G 8 + X 8
 #   column pass (forward cols, modulus, inverse cols) : read X 8 + M 4, write X 8
 KERNEL_BYTES = {"row_mid": 33, "col_proj": 20, "blur": 8, "row_last": 21}
@@ -72,7 +72,7 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def cpu_baseline(M: np.ndarray, budget_s: float = 12.0, max_iter: int = 20) -> dict:
+def cpu_baseline(M: np.ndarray, M256: np.ndarray, budget_s: float = 12.0, max_iter: int = 20) -> dict:
     import oracle  # CPU checker; only the baseline leg uses it
 
     oracle.set_fft_threads(1)   # the reference plans FFTW single-threaded
@@ -84,12 +84,19 @@ def cpu_baseline(M: np.ndarray, budget_s: float = 12.0, max_iter: int = 20) -> d
         g, gp = oracle.hio_iterations(g, gp, mask, M, 0.9, 1, np.float32)
         done += 1
     dt = time.perf_counter() - t0
+    # the reference's CPU-runnable case: one full 20 x 20 reconstruction of the
+    # 256 x 256 workload (shrinkWrap.cpp:111-361 restated)
+    t1 = time.perf_counter()
+    oracle.shrink_wrap(M256, n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
+    dt256 = time.perf_counter() - t1
     return {"value": round(done / dt, 4), "unit": "HIO iters/s", "cores": oracle.num_threads(),
             "kind": "port",
             "sample": f"{done} HIO iterations (constraint, FFT, modulus, IFFT) of the 2048x2048 "
                       f"workload on the CPU oracle; FFT single-threaded as the reference, "
                       f"elementwise loops on {oracle.num_threads()} OpenMP threads; "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s",
+            "recon_256_s": round(dt256, 3),
+            "recon_256_note": "one full 20x20 reconstruction of the 256x256 workload on the CPU oracle"}
 
 
 def _gen_batch_input(args):
@@ -319,6 +326,31 @@ def large_bench(L, M8, steps: int = 2) -> dict:
             "iteration_alg_GBps_80B": round(80 * n * n * N_CYCLES * N_HIO / dt / 1e9, 1)}
 
 
+def small_bench(L, M256: np.ndarray, reps: int = 20) -> dict:
+    """The reference's CPU-runnable case (256 x 256, 8 shapes, no noise,
+    20 x 20) on the GPU: reconstructions/s of back-to-back calls with the
+    modulus resident (restored by a device copy before each)."""
+    n = M256.shape[0]
+    d_src = im.DeviceBuffer.from_array(M256)
+    d_io = im.DeviceBuffer(M256.nbytes)
+    p = im.default_params(n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
+
+    def step():
+        im.check(L.imresh_memcpy_d2d(d_io.ptr, d_src.ptr, M256.nbytes, None), "d2d")
+        im.check(L.imresh_shrinkwrap_device(d_io.ptr, n, n, p, None, None), "shrinkwrap 256")
+
+    step()
+    im.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    im.sync()
+    dt = (time.perf_counter() - t0) / reps
+    return {"workload": "shrink-wrap 256x256 fp32, 20 cycles x 20 HIO, 8 shapes in central 128^2",
+            "recon_per_s": round(1.0 / dt, 1), "hio_iters_per_s": round(N_CYCLES * N_HIO / dt, 1),
+            "ms_per_reconstruction": round(dt * 1e3, 3)}
+
+
 def forward_model_bench(L, obj: np.ndarray, reps: int = 50) -> dict:
     """The device forward model |FFT2(object)| (imresh_diffraction_intensity_device)
     at the object's size: kernel-exact durations of its two fused passes
@@ -380,15 +412,17 @@ def main() -> int:
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-fft-compare", action="store_true")
     ap.add_argument("--no-forward-model", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): independent images shard "
-              f"across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+              f"across GPUs with no communication (one engine per GPU)", file=sys.stderr)
         return 2
 
     log("[bench] generating the 2048x2048 workload")
     obj, M = syn.config_2048(0)
+    M256 = syn.config_256(0)[1]
     M8 = None
     if not args.no_large:
         log("[bench] generating the 8192x8192 workload")
@@ -421,11 +455,20 @@ def main() -> int:
     value = iters / dt
     ms_per_step = 1000.0 * dt / args.steps
 
-    # quality check of the timed output (not part of the timing)
+    # quality check of the timed output (not part of the timing): HIO error
+    # of one more identical reconstruction and NCC of the object estimate to
+    # the known synthetic object (modulo translation and twin image)
     out = d_io.download(np.float32, M.shape)
     if not np.all(np.isfinite(out)):
         log("[bench] non-finite reconstruction")
         return 1
+    res = im.Result()
+    im.check(L.imresh_memcpy_d2d(d_io.ptr, d_src.ptr, M.nbytes, None), "d2d")
+    im.check(L.imresh_shrinkwrap_device(d_io.ptr, W, H, p, None, im.ctypes.byref(res)), "shrinkwrap")
+    out2 = d_io.download(np.float32, M.shape)
+    quality = {"hio_error": float(res.last_error), "cycles_run": int(res.cycles_run),
+               "ncc_to_object": round(syn.ncc(out, obj), 6),
+               "bitwise_repeatable": bool(np.array_equal(out, out2))}
 
     log("[bench] per-kernel durations (kernel-exact HIP events)")
     ms = (im.ctypes.c_float * 4)()
@@ -445,8 +488,12 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
                 "kernel": dom, "bytes_per_launch": KERNEL_BYTES[dom] * n, "kernels": kernels}
-    # whole-iteration view: the reference's operator sequence moves 80 B/element per
-    # HIO iteration (76 + 4-byte mask); the fused path moves 53 B/element
+    # whole-iteration view: the reference's operator sequence moves 76 + m
+    # B/element per HIO iteration, m = bytes per mask element: 77 with this
+    # build's byte mask, 80 with the reference's float mask; the fused path
+    # moves 53 B/element
+    roofline["iteration_alg_GBps_77B"] = round(77 * n * value / 1e9, 1)
+    roofline["iteration_frac_77B"] = round(77 * n * value / 1e9 / HBM_PEAK_GBS, 4)
     roofline["iteration_alg_GBps_80B"] = round(80 * n * value / 1e9, 1)
 
     batch = None
@@ -465,6 +512,11 @@ def main() -> int:
         large = large_bench(L, M8)
         del M8
 
+    small = None
+    if not args.no_small:
+        log("[bench] 256x256 reconstructions")
+        small = small_bench(L, M256)
+
     fwd = None
     if not args.no_forward_model:
         log("[bench] forward model |FFT2(object)|")
@@ -479,7 +531,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
-        cpu = cpu_baseline(M)
+        cpu = cpu_baseline(M, M256)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "HIO iters/s", "n_gpus": 1,
@@ -490,7 +542,8 @@ def main() -> int:
                                "sigma0 3.0, cut 0.2/0.04, 64 shapes in central 1024^2",
                    "iterations_per_step": N_CYCLES * N_HIO, "width": W, "height": H,
                    "row_variant": rv.value, "col_variant": cv.value},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batch,
+        "roofline": roofline, "cpu_baseline": cpu, "quality": quality, "small_256": small,
+        "batched": batch,
         "serving": serve, "large_8192": large, "fft_vs_rocfft": fftc, "forward_model": fwd,
     }
     print(json.dumps(line), flush=True)

@@ -118,6 +118,9 @@ _SIGS = {
                                            POINTER(c_float), POINTER(c_float), _P]),
     "imresh_calculate_hio_error_u8": (c_int, [_P, _P, c_uint32, c_int, POINTER(c_float),
                                               POINTER(c_float), POINTER(c_float), _P]),
+    "imresh_calculate_hio_error_bits": (c_int, [_P, _P, c_uint32, c_int, POINTER(c_float),
+                                                POINTER(c_float), POINTER(c_float), _P]),
+    "imresh_pack_mask": (c_int, [_P, c_uint32, _P, _P]),
     "imresh_gaussian_kernel": (c_int, [c_double, _P, c_uint32]),
     "imresh_gaussian_blur": (c_int, [_P, c_uint32, c_uint32, c_double, _P]),
     "imresh_gaussian_blur_horizontal": (c_int, [_P, c_uint32, c_uint32, c_double, _P]),

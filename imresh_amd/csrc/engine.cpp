@@ -275,6 +275,10 @@ int time_variant(bool row, int var, Plan2D p, Workspace *w, hipStream_t s, float
     ra.thr = w->scal; ra.beta = 0.9f; ra.perr = w->perr; ra.pcnt = w->pcnt;
     ColArgs ca{};
     ca.X = w->X; ca.M = w->M;
+    ca.in = w->X; ca.out = w->X;
+    // a length whose projection pass is the four-step one uses the column
+    // variant only for its plain transforms: time those
+    const int col_mode = p.col.fs_h2 > 0 ? COL_FWD : COL_PROJ;
     if (row) p.rowVar = var; else p.colVar = var;
     hipEvent_t e0, e1;
     IMR_HIP_CHECK(hipEventCreate(&e0));
@@ -288,7 +292,7 @@ int time_variant(bool row, int var, Plan2D p, Workspace *w, hipStream_t s, float
         st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, w->W, s);
         for (int r = 0; r <= reps && st == IMRESH_OK; ++r) {
             if (r == 1) hipEventRecord(e0, s);   // first launch warms up
-            st = row ? launch_row(ROW_MID, p, ra, s) : launch_col(COL_PROJ, p, ca, s);
+            st = row ? launch_row(ROW_MID, p, ra, s) : launch_col(col_mode, p, ca, s);
         }
         hipEventRecord(e1, s);
         if (st == IMRESH_OK) st = from_hip(hipEventSynchronize(e1));
@@ -577,7 +581,7 @@ int vector_reduce(int op, const float *d, uint32_t n, float *out, void *stream)
     return read_scalar(res, out, s);
 }
 
-int hio_error(const float *g, const void *mask, int u8, uint32_t n, int invert, float *err,
+int hio_error(const float *g, const void *mask, int kind, uint32_t n, int invert, float *err,
               float *tot, float *nm, void *stream)
 {
     if (g == nullptr || mask == nullptr || err == nullptr) return IMRESH_ERR_NULL;
@@ -587,7 +591,7 @@ int hio_error(const float *g, const void *mask, int u8, uint32_t n, int invert, 
     float *perr = (float *) sc.p;
     unsigned *pcnt = (unsigned *) (perr + kReduceBlocks);
     float *res = perr + 2 * kReduceBlocks;
-    IMR_CHECK(launch_hio_error((const float2 *) g, mask, u8, n, invert, perr, pcnt, kReduceBlocks,
+    IMR_CHECK(launch_hio_error((const float2 *) g, mask, kind, n, invert, perr, pcnt, kReduceBlocks,
                                res, s));
     float h[3];
     IMR_HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1085,13 +1089,25 @@ int imresh_vector_sum(const float *d, uint32_t n, float *r, void *s) { return ve
 int imresh_calculate_hio_error(const float *g, const float *mask, uint32_t n, int invert,
                                float *err, float *tot, float *nm, void *stream)
 {
-    return hio_error(g, mask, 0, n, invert, err, tot, nm, stream);
+    return hio_error(g, mask, kMaskFloat, n, invert, err, tot, nm, stream);
 }
 
 int imresh_calculate_hio_error_u8(const float *g, const uint8_t *mask, uint32_t n, int invert,
                                   float *err, float *tot, float *nm, void *stream)
 {
-    return hio_error(g, mask, 1, n, invert, err, tot, nm, stream);
+    return hio_error(g, mask, kMaskU8, n, invert, err, tot, nm, stream);
+}
+
+int imresh_calculate_hio_error_bits(const float *g, const uint32_t *mask_bits, uint32_t n, int invert,
+                                    float *err, float *tot, float *nm, void *stream)
+{
+    return hio_error(g, mask_bits, kMaskBits, n, invert, err, tot, nm, stream);
+}
+
+int imresh_pack_mask(const uint8_t *mask_dev, uint32_t n, uint32_t *bits_dev, void *stream)
+{
+    if (mask_dev == nullptr || bits_dev == nullptr) return IMRESH_ERR_NULL;
+    return launch_pack_mask(mask_dev, n, bits_dev, S(stream));
 }
 
 int imresh_gaussian_kernel(double sigma, float *w, uint32_t capacity)

@@ -62,7 +62,7 @@ struct ColArgs {
     float2 *X;
     const float *M;        // modulus (COL_PROJ) or real input (COL_INIT)
     const int *done;
-    int flags;             // kCol* bits (set by launch_col from the mode)
+    int flags;             // unused by the column kernels (kept 0)
     int W;                 // number of columns = row pitch
     const float2 *tw;      // column-length twiddle table (set by launch_col)
     int batch;             // images (grid.y; 0 or 1: one image)
@@ -216,8 +216,12 @@ int launch_cutoff(float *d, size_t n, float t, float lo, float hi, hipStream_t s
 // op: 0 = max, 1 = min, 2 = sum.  result: device float.
 int launch_reduce(int op, const float *d, size_t n, float *partials, int np, float *result,
                   hipStream_t s);
-int launch_hio_error(const float2 *g, const void *mask, int mask_is_u8, size_t n, int invert,
+// mask kinds of launch_hio_error
+constexpr int kMaskFloat = 0, kMaskU8 = 1, kMaskBits = 2;
+int launch_hio_error(const float2 *g, const void *mask, int mask_kind, size_t n, int invert,
                      float *perr, unsigned *pcnt, int np, float *res, hipStream_t s);
+// byte mask -> bit-packed mask (32 elements per word, element 32w+j at bit 31-j)
+int launch_pack_mask(const uint8_t *mask, size_t n, uint32_t *bits, hipStream_t s);
 int launch_fill_state(float2 *X, float2 *G, uint8_t *mask, float *M, float *Bn, size_t n,
                       int W, hipStream_t s);
 

@@ -6,11 +6,10 @@
 //            Finishes the forward 2-D FFT of an HIO iteration, replaces |G'|
 //            by the measured modulus (cudaVectorElementwise.cu:168-195) in
 //            registers, and starts the inverse 2-D FFT.
-//   CK_INV   inverse column transform; with kColRealIn the input is the real
-//            modulus M, i.e. the first half of IFFT2((M, 0))
-//            (cudaShrinkWrap.cu:229-232).
-//   CK_FWD   forward column transform of the 2-D FFT entry point; with
-//            kColRealIn the input is a real plane, the first half of the
+//   CK_INV   inverse column transform; CK_INV_REAL reads the real modulus M
+//            instead, the first half of IFFT2((M, 0)) (cudaShrinkWrap.cu:229-232).
+//   CK_FWD   forward column transform of the 2-D FFT entry point;
+//            CK_FWD_REAL reads a real plane instead, the first half of the
 //            forward model FFT2((x, 0)) (diffractionIntensity.cpp:59-70).
 #include "internal.hpp"
 #include "blue_device.hpp"
@@ -27,7 +26,7 @@ namespace imr {
 
 namespace {
 
-enum ColKernel { CK_FWD, CK_INV, CK_PROJ };
+enum ColKernel { CK_FWD, CK_INV, CK_PROJ, CK_FWD_REAL, CK_INV_REAL };
 
 // C columns per workgroup; SPLIT: exchange real and imaginary parts through
 // LDS one after the other (half the LDS of a complex tile); PF (CK_PROJ
@@ -72,22 +71,20 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
     cf v[P::E];
     const Blue bl{BL ? a.len : P::n, (const cf *) a.bs};
 
-    if constexpr (K == CK_FWD) {
-        const bool real_in = (a.flags & kColRealIn) != 0;
-        const cf *in = (const cf *) a.in + wb;
-        const float *M = a.M + wb;
+    if constexpr (K == CK_FWD || K == CK_INV || K == CK_FWD_REAL || K == CK_INV_REAL) {
+        constexpr bool real_in = K == CK_FWD_REAL || K == CK_INV_REAL;
+        constexpr int DIR = (K == CK_FWD || K == CK_FWD_REAL) ? -1 : +1;
         cf *out = (cf *) a.out + wb;
-        auto g0 = [=](int e) { return real_in ? mkc(at32(M, e * W + co), 0.f) : at32(in, e * W + co); };
         auto sl = [=](int e, cf x) { if (valid) at32(out, e * W + co) = x; };
-        xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
-    } else if constexpr (K == CK_INV) {
-        const bool real_in = (a.flags & kColRealIn) != 0;
-        const cf *in = (const cf *) a.in + wb;
-        const float *M = a.M + wb;
-        cf *out = (cf *) a.out + wb;
-        auto g0 = [=](int e) { return real_in ? mkc(at32(M, e * W + co), 0.f) : at32(in, e * W + co); };
-        auto sl = [=](int e, cf x) { if (valid) at32(out, e * W + co) = x; };
-        xfft<P, +1, BL>(v, tw, t, L, bl, g0, sl);
+        if constexpr (real_in) {
+            const float *M = a.M + wb;
+            auto g0 = [=](int e) { return mkc(at32(M, e * W + co), 0.f); };
+            xfft<P, DIR, BL>(v, tw, t, L, bl, g0, sl);
+        } else {
+            const cf *in = (const cf *) a.in + wb;
+            auto g0 = [=](int e) { return at32(in, e * W + co); };
+            xfft<P, DIR, BL>(v, tw, t, L, bl, g0, sl);
+        }
     } else if constexpr (K == CK_PROJ) {
         cf *X = (cf *) a.X + wb;
         const float *M = a.M + mb;
@@ -122,6 +119,8 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
             hipError_t e = hipSuccess;
             for (const void *f : {(const void *) k_col<P, C, SPLIT, CK_FWD, false, BL>,
                                   (const void *) k_col<P, C, SPLIT, CK_INV, false, BL>,
+                                  (const void *) k_col<P, C, SPLIT, CK_FWD_REAL, false, BL>,
+                                  (const void *) k_col<P, C, SPLIT, CK_INV_REAL, false, BL>,
                                   (const void *) k_col<P, C, SPLIT, CK_PROJ, PF, BL, STG>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
@@ -135,19 +134,17 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
         break;
     case COL_REAL_FWD:
-        a.flags = kColRealIn;
-        a.in = a.out;   // unused with kColRealIn; never a null pointer in a kernel
-        IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
+        a.flags = 0;
+        IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD_REAL, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INV:
         a.flags = 0;
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INIT:
-        a.flags = kColRealIn;
-        a.in = a.X;   // unused with kColRealIn; never a null pointer in a kernel
+        a.flags = 0;
         a.out = a.X;
-        IMR_LAUNCH((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_col<P, C, SPLIT, CK_INV_REAL, false, BL>), grid, block, lds, s, a);
         break;
     case COL_PROJ:
         a.flags = 0;

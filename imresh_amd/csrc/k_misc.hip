@@ -303,7 +303,22 @@ __global__ void __launch_bounds__(1024) k_reduce_final(const float *partials, in
     if (threadIdx.x == 0) *result = acc;
 }
 
-// HIO error partial sums with a float or unsigned char mask
+// mask element i: a float or unsigned char per element, or bit-packed
+// (MaskBits: 32 elements per word, element i at bit 31 - i % 32 of word
+// i / 32, the layout of cudaKernelCalculateHioErrorBitPacked,
+// benchmark/imresh/algorithms/cuda/cudaVectorReduce.cu:402-458)
+struct MaskBits {
+    uint32_t w;
+};
+template <class MT>
+__device__ __forceinline__ bool mask_at(const MT *m, size_t i) { return m[i] != 0; }
+template <>
+__device__ __forceinline__ bool mask_at<MaskBits>(const MaskBits *m, size_t i)
+{
+    return ((m[i >> 5].w >> (31u - (unsigned) (i & 31))) & 1u) != 0;
+}
+
+// HIO error partial sums with a float, unsigned char or bit-packed mask
 // (cudaVectorReduce.cu:284-355: shouldBeZero = invert xor (bool) mask)
 template <class MT>
 __global__ void __launch_bounds__(256) k_hio_error(const float2 *g, const MT *mask, size_t n,
@@ -315,7 +330,7 @@ __global__ void __launch_bounds__(256) k_hio_error(const float2 *g, const MT *ma
     unsigned c = 0;
     for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n;
          i += (size_t) gridDim.x * blockDim.x) {
-        const bool z = (invert != 0) != (mask[i] != 0);
+        const bool z = (invert != 0) != mask_at(mask, i);
         if (z) {
             s += cnorm(((const cf *) g)[i]);
             c += 1;
@@ -640,17 +655,44 @@ int launch_reduce(int op, const float *d, size_t n, float *partials, int np, flo
     return from_hip(hipGetLastError());
 }
 
-int launch_hio_error(const float2 *g, const void *mask, int mask_is_u8, size_t n, int invert,
+int launch_hio_error(const float2 *g, const void *mask, int mask_kind, size_t n, int invert,
                      float *perr, unsigned *pcnt, int np, float *res, hipStream_t s)
 {
-    if (mask_is_u8)
+    if (mask_kind == kMaskU8)
         hipLaunchKernelGGL(k_hio_error<uint8_t>, dim3(np), dim3(256), 0, s, g,
                            (const uint8_t *) mask, n, invert, perr, pcnt);
+    else if (mask_kind == kMaskBits)
+        hipLaunchKernelGGL(k_hio_error<MaskBits>, dim3(np), dim3(256), 0, s, g,
+                           (const MaskBits *) mask, n, invert, perr, pcnt);
     else
         hipLaunchKernelGGL(k_hio_error<float>, dim3(np), dim3(256), 0, s, g, (const float *) mask,
                            n, invert, perr, pcnt);
     hipLaunchKernelGGL(k_err_finalize, dim3(1), dim3(1024), 0, s, perr, pcnt, np, 0.f, res,
                        (int *) nullptr, (unsigned *) nullptr);
+    return from_hip(hipGetLastError());
+}
+
+// word w of the bit-packed mask from elements 32w .. 32w+31 of a byte mask
+// (nonzero = 1), element 32w + j at bit 31 - j; a partial last word is
+// zero-filled
+__global__ void __launch_bounds__(256) k_pack_mask(const uint8_t *mask, size_t n, uint32_t *bits)
+{
+    const size_t nw = (n + 31) / 32;
+    for (size_t w = blockIdx.x * (size_t) blockDim.x + threadIdx.x; w < nw;
+         w += (size_t) gridDim.x * blockDim.x) {
+        uint32_t v = 0;
+        const size_t i0 = w * 32;
+        for (int j = 0; j < 32; ++j)
+            if (i0 + j < n && mask[i0 + j] != 0) v |= 1u << (31 - j);
+        bits[w] = v;
+    }
+}
+
+int launch_pack_mask(const uint8_t *mask, size_t n, uint32_t *bits, hipStream_t s)
+{
+    const size_t nw = (n + 31) / 32;
+    if (nw == 0) return IMRESH_OK;
+    hipLaunchKernelGGL(k_pack_mask, dim3(grid_1d(nw)), dim3(256), 0, s, mask, n, bits);
     return from_hip(hipGetLastError());
 }
 

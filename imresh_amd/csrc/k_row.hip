@@ -13,8 +13,8 @@
 //   RK_INVOUT inverse row FFT closing a reconstruction step: writes g' and,
 //             by flags, gPrev = g' (cudaShrinkWrap.cu:304-309), |g'| for the
 //             blur (:240) and the HIO error sums (cudaVectorReduce.cu:284-355).
-//   RK_FWD / RK_INV  plain row transforms of the 2-D FFT entry point; with
-//            kRowNormOut the forward one stores |z| (the forward model).
+//   RK_FWD / RK_INV  plain row transforms of the 2-D FFT entry point;
+//            RK_FWD_NORM stores |z| of the forward one (the forward model).
 #include "internal.hpp"
 #include "blue_device.hpp"
 #include "ops_device.hpp"
@@ -28,7 +28,7 @@ namespace imr {
 
 namespace {
 
-enum RowKernel { RK_FWD, RK_INV, RK_INVOUT, RK_FIRST, RK_MID };
+enum RowKernel { RK_FWD, RK_INV, RK_INVOUT, RK_FIRST, RK_MID, RK_FWD_NORM };
 
 // NL rows per workgroup; PF (RK_MID only): gPrev and the mask are read right
 // after the row gather instead of at the junction; BL: Bluestein kernels of
@@ -63,24 +63,19 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     auto ldX = [=](int e) { return at32(X, lo + e); };
     auto stX = [=](int e, cf x) { if (valid) at32(X, lo + e) = x; };
 
-    if constexpr (K == RK_FWD || K == RK_INV) {
+    if constexpr (K == RK_FWD || K == RK_INV || K == RK_FWD_NORM) {
         const cf *in = (const cf *) a.in + wb;
-        cf *out = (cf *) a.out + wb;
         auto g0 = [=](int e) { return at32(in, lo + e); };
-        if constexpr (K == RK_FWD) {
-            // kRowNormOut: the last half of the forward model |FFT2(x)|
-            // (diffractionIntensity.cpp:72-77), only |z| leaves the registers
-            const bool norm = (a.flags & kRowNormOut) != 0;
+        if constexpr (K == RK_FWD_NORM) {
+            // the last half of the forward model |FFT2(x)|
+            // (diffractionIntensity.cpp:72-77): only |z| leaves the registers
             float *Bn = a.Bn + wb;
-            auto sl = [=](int e, cf x) {
-                if (!valid) return;
-                if (norm) at32(Bn, lo + e) = cnorm(x);
-                else at32(out, lo + e) = x;
-            };
+            auto sl = [=](int e, cf x) { if (valid) at32(Bn, lo + e) = cnorm(x); };
             xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
         } else {
+            cf *out = (cf *) a.out + wb;
             auto sl = [=](int e, cf x) { if (valid) at32(out, lo + e) = x; };
-            xfft<P, +1, BL>(v, tw, t, L, bl, g0, sl);
+            xfft<P, K == RK_FWD ? -1 : +1, BL>(v, tw, t, L, bl, g0, sl);
         }
     } else if constexpr (K == RK_INVOUT) {
         __shared__ float red_f[NL * P::T];
@@ -168,6 +163,7 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         static const hipError_t attr = [&] {
             hipError_t e = hipSuccess;
             for (const void *f : {(const void *) k_row<P, NL, RK_FWD, false, BL>,
+                                  (const void *) k_row<P, NL, RK_FWD_NORM, false, BL>,
                                   (const void *) k_row<P, NL, RK_INV, false, BL>,
                                   (const void *) k_row<P, NL, RK_INVOUT, false, BL>,
                                   (const void *) k_row<P, NL, RK_FIRST, false, BL>,
@@ -181,9 +177,8 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
     switch (mode) {
     case ROW_FWD: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
     case ROW_FWD_NORM:
-        a.flags = kRowNormOut;
-        a.out = (float2 *) a.Bn;   // unused with kRowNormOut; never a null pointer in a kernel
-        IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a);
+        a.flags = 0;
+        IMR_LAUNCH((k_row<P, NL, RK_FWD_NORM, false, BL>), grid, block, lds, s, a);
         break;
     case ROW_INV: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_INV, false, BL>), grid, block, lds, s, a); break;
     case ROW_INIT:

@@ -126,8 +126,5 @@ constexpr int kRowWriteG = 1;     // write gPrev = g'
 constexpr int kRowWriteBn = 2;    // write |g'| (blur input)
 constexpr int kRowError = 4;      // per-workgroup HIO error sums
 constexpr int kRowCut = 8;        // mask = blurred < threshold (else read mask)
-constexpr int kRowNormOut = 16;   // forward transform stores |z| into Bn instead of z
-// ColArgs.flags
-constexpr int kColRealIn = 1;     // input is a real plane (M) instead of complex
 
 }  // namespace imr

@@ -206,7 +206,12 @@ int get_line_plan(int device, int n, LinePlan *out)
     lp.C = pi->C;
     lp.tw = dev;
     if (n >= kFourStepMinLine && n % 8 == 0 && find_info(n / 8) != nullptr) {
-        // four-step column pass: the twiddles exp(-2 pi i j / n) of its outer steps
+        // four-step column pass: its length-n/8 plan (created now: the pass
+        // may first run inside a graph capture, where no allocation or copy
+        // may happen) and the twiddles exp(-2 pi i j / n) of its outer steps
+        LinePlan sub;
+        hipSetDevice(prev);
+        IMR_CHECK(get_line_plan(device, n / 8, &sub));
         std::vector<float2> t(n);
         for (int j = 0; j < n; ++j) {
             const double ang = -2.0 * M_PI * (double) j / (double) n;

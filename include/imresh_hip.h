@@ -242,6 +242,15 @@ int imresh_calculate_hio_error(const float *gprime_dev, const float *mask_dev, u
 int imresh_calculate_hio_error_u8(const float *gprime_dev, const uint8_t *mask_dev, uint32_t n,
                                   int invert_mask, float *error_host, float *total_host,
                                   float *nmasked_host, void *stream);
+/* The same with a bit-packed mask (cudaCalculateHioErrorBitPacked,
+ * benchmark/imresh/algorithms/cuda/cudaVectorReduce.cu:402-505): 32 elements
+ * per uint32 word, element i at bit 31 - i%32 of word i/32. */
+int imresh_calculate_hio_error_bits(const float *gprime_dev, const uint32_t *mask_bits_dev, uint32_t n,
+                                    int invert_mask, float *error_host, float *total_host,
+                                    float *nmasked_host, void *stream);
+/* Byte mask (nonzero = 1) -> that bit-packed layout, ceil(n/32) words, a
+ * partial last word zero-filled.  Asynchronous. */
+int imresh_pack_mask(const uint8_t *mask_dev, uint32_t n, uint32_t *bits_dev, void *stream);
 /* calcGaussianKernel, calcGaussianKernel.cpp:42-91 (host).  Returns the tap
  * count; writes the weights when weights != NULL and capacity suffices. */
 int imresh_gaussian_kernel(double sigma, float *weights_host, uint32_t capacity);

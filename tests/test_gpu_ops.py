@@ -400,3 +400,66 @@ def test_device_forward_model_feeds_reconstruction():
     im.check(L().imresh_shrinkwrap_device(d.ptr, 128, 96, p, None, r2))
     assert np.array_equal(d.download(np.float32, (96, 128)), host)
     assert r.cycles_run == r2.cycles_run and r.last_error == r2.last_error
+
+
+# ---------------------------------------------------------------- bit-packed mask
+def _unpack_msb_first(words, n):
+    """unpackBitMask (testVectorReduce.cu:199-228): element i = bit 31 - i%32 of word i/32."""
+    b = np.unpackbits(np.asarray(words, ">u4").view(np.uint8), bitorder="big")
+    return b[:n]
+
+
+def test_pack_mask_layout():
+    """0x33333333 unpacks to 0,0,1,1,... (testVectorReduce.cu:230-246); the
+    packing kernel is its inverse, a partial last word zero-filled."""
+    assert list(_unpack_msb_first([0x33333333], 30)) == [(i // 2) % 2 for i in range(30)]
+    rng = np.random.default_rng(4)
+    for n in (1, 31, 32, 33, 1000, 4097):
+        m = (rng.random(n) < 0.5).astype(np.uint8) * rng.integers(1, 255, n).astype(np.uint8)
+        dm, db = B.from_array(m), B(4 * ((n + 31) // 32))
+        im.check(L().imresh_pack_mask(dm.ptr, n, db.ptr, None))
+        words = db.download(np.uint32, ((n + 31) // 32,))
+        assert np.array_equal(_unpack_msb_first(words, n), (m != 0).astype(np.uint8))
+        tail = (n % 32)
+        if tail:
+            assert words[-1] & ((1 << (32 - tail)) - 1) == 0
+
+
+def test_hio_error_bit_packed_pythagorean():
+    """testVectorReduce.cu:248-410: random packed words (srand(350471643),
+    rand() % UINT_MAX), masked elements 3+4i and unmasked random ones; for
+    log-spaced prefix lengths the bit-packed error gives total = 5 * count
+    while exact, the count of the unpacked mask, and equals the unsigned char
+    and float mask versions."""
+    n_max = 1 << 20
+    nw = (n_max + 31) // 32
+    srand(350471643)
+    words = np.array([rand() % 0xFFFFFFFF for _ in range(nw)], np.uint32)
+    mask = _unpack_msb_first(words, n_max)
+    data = np.empty(n_max, np.complex64)
+    idx = np.flatnonzero(mask == 0)
+    data[mask == 1] = 3 + 4j
+    r = np.array([rand() for _ in range(2 * idx.size)], np.float64) / RAND_MAX
+    data[idx] = (r[0::2].astype(np.float32) + 1j * r[1::2].astype(np.float32)).astype(np.complex64)
+    dd, db = B.from_array(data), B.from_array(words)
+    du, df = B.from_array(mask.astype(np.uint8)), B.from_array(mask.astype(np.float32))
+    err, tot, cnt = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    last = 0.0
+    for n in log_spaced_points(2, n_max, 50):
+        im.check(L().imresh_calculate_hio_error_bits(dd.ptr, db.ptr, n, 0, ctypes.byref(err),
+                                                     ctypes.byref(tot), ctypes.byref(cnt), None))
+        exp_cnt = float(mask[:n].sum())
+        assert cnt.value == exp_cnt and cnt.value >= last
+        last = cnt.value
+        if tot.value < 16777216:
+            assert tot.value == 5 * cnt.value
+        got = (err.value, tot.value, cnt.value)
+        for dm, fn in ((du, "imresh_calculate_hio_error_u8"), (df, "imresh_calculate_hio_error")):
+            e2, t2, c2 = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+            im.check(getattr(L(), fn)(dd.ptr, dm.ptr, n, 0, ctypes.byref(e2), ctypes.byref(t2),
+                                      ctypes.byref(c2), None))
+            assert (e2.value, t2.value, c2.value) == got, (n, fn)
+    # the inverted mask counts the complement
+    im.check(L().imresh_calculate_hio_error_bits(dd.ptr, db.ptr, 1000, 1, ctypes.byref(err),
+                                                 ctypes.byref(tot), ctypes.byref(cnt), None))
+    assert cnt.value == 1000 - mask[:1000].sum()
