@@ -47,7 +47,7 @@ struct RowArgs {
     float *perr;           // per-workgroup sum mask*|g'|
     unsigned *pcnt;        // per-workgroup sum mask
     const int *done;       // early-exit flag (NULL: never)
-    int flags;             // kRow* bits (set by launch_row from the mode)
+    int flags;             // unused by the row kernels (kept 0: the mode selects the kernel)
     int H;                 // number of rows (of one image)
     const float2 *tw;      // row-length twiddle table (set by launch_row)
     int batch;             // images (grid.y; 0 or 1: one image)

@@ -32,8 +32,10 @@ enum RowKernel { RK_FWD, RK_INV, RK_INVOUT, RK_FIRST, RK_MID, RK_FWD_NORM };
 
 // NL rows per workgroup; PF (RK_MID only): gPrev and the mask are read right
 // after the row gather instead of at the junction; BL: Bluestein kernels of
-// row length a.len with P the convolution plan
-template <class P, int NL, int K, bool PF = false, bool BL = false, int STG = 0>
+// row length a.len with P the convolution plan; FL (RK_INVOUT, RK_FIRST): the
+// kRow* outputs / cut-off, fixed at compile time (a uniform runtime flag
+// tested per element slows the memory phases measurably)
+template <class P, int NL, int K, bool PF = false, bool BL = false, int STG = 0, int FL = 0>
 __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
 {
     // image b of a batch (grid.y): its planes start b * istride elements in,
@@ -80,7 +82,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     } else if constexpr (K == RK_INVOUT) {
         __shared__ float red_f[NL * P::T];
         __shared__ unsigned red_u[NL * P::T];
-        const int fl = a.flags;
+        constexpr int fl = FL;
         cf *G = (cf *) a.G + wb;
         float *Bn = a.Bn + wb;
         const uint8_t *mask = a.mask + wb;
@@ -89,10 +91,10 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         auto sl = [&](int e, cf g) {
             if (!valid) return;
             at32(X, lo + e) = g;
-            if (fl & kRowWriteG) at32(G, lo + e) = g;
-            if (fl & (kRowWriteBn | kRowError)) {
+            if constexpr ((fl & kRowWriteG) != 0) at32(G, lo + e) = g;
+            if constexpr ((fl & (kRowWriteBn | kRowError)) != 0) {
                 const float nrm = cnorm(g);
-                if (fl & kRowWriteBn) at32(Bn, lo + e) = nrm;
+                if constexpr ((fl & kRowWriteBn) != 0) at32(Bn, lo + e) = nrm;
                 if ((fl & kRowError) && at32(mask, lo + e)) {
                     err += nrm;
                     cnt += 1;
@@ -100,7 +102,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             }
         };
         xfft<P, +1, BL>(v, tw, t, L, bl, ldX, sl);
-        if (fl & kRowError) {
+        if constexpr ((fl & kRowError) != 0) {
             err = block_reduce(err, red_f, [](float x, float y) { return x + y; });
             cnt = block_reduce(cnt, red_u, [](unsigned x, unsigned y) { return x + y; });
             if (threadIdx.x == 0) {
@@ -109,7 +111,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             }
         }
     } else if constexpr (K == RK_FIRST) {
-        const bool cut = (a.flags & kRowCut) != 0;
+        constexpr bool cut = (FL & kRowCut) != 0;
         const float thr = cut ? a.thr[b * kScalStride] : 0.f;
         cf *G = (cf *) a.G + wb;
         uint8_t *mask = a.mask + wb;
@@ -118,7 +120,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
         auto g0 = [=](int e) {
             const cf g = at32(X, lo + e);
             bool m;
-            if (cut) {
+            if constexpr (cut) {
                 m = at32(B2, lo + e) < thr;
                 if (valid) at32(mask, lo + e) = m ? 1 : 0;
             } else {
@@ -152,6 +154,11 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
     }
 }
 
+// the output sets of the closing row pass
+constexpr int kInit = kRowWriteG | kRowWriteBn;
+constexpr int kLast = kRowWriteBn | kRowError;
+constexpr int kInitErr = kRowWriteG | kRowWriteBn | kRowError;
+
 template <class P, int NL, bool PF = false, bool BL = false, int STG = 0>
 int launch_variant(int mode, RowArgs a, hipStream_t s)
 {
@@ -165,8 +172,12 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
             for (const void *f : {(const void *) k_row<P, NL, RK_FWD, false, BL>,
                                   (const void *) k_row<P, NL, RK_FWD_NORM, false, BL>,
                                   (const void *) k_row<P, NL, RK_INV, false, BL>,
-                                  (const void *) k_row<P, NL, RK_INVOUT, false, BL>,
-                                  (const void *) k_row<P, NL, RK_FIRST, false, BL>,
+                                  (const void *) k_row<P, NL, RK_INVOUT, false, BL, 0, kInit>,
+                                  (const void *) k_row<P, NL, RK_INVOUT, false, BL, 0, kLast>,
+                                  (const void *) k_row<P, NL, RK_INVOUT, false, BL, 0, 0>,
+                                  (const void *) k_row<P, NL, RK_INVOUT, false, BL, 0, kInitErr>,
+                                  (const void *) k_row<P, NL, RK_FIRST, false, BL, 0, kRowCut>,
+                                  (const void *) k_row<P, NL, RK_FIRST, false, BL, 0, 0>,
                                   (const void *) k_row<P, NL, RK_MID, PF, BL, STG>})
                 if (e == hipSuccess)
                     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
@@ -174,38 +185,20 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         }();
         if (attr != hipSuccess) return from_hip(attr);
     }
+    a.flags = 0;
     switch (mode) {
-    case ROW_FWD: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
-    case ROW_FWD_NORM:
-        a.flags = 0;
-        IMR_LAUNCH((k_row<P, NL, RK_FWD_NORM, false, BL>), grid, block, lds, s, a);
-        break;
-    case ROW_INV: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_INV, false, BL>), grid, block, lds, s, a); break;
-    case ROW_INIT:
-        a.flags = kRowWriteG | kRowWriteBn;
-        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
-        break;
-    case ROW_LAST:
-        a.flags = kRowWriteBn | kRowError;
-        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
-        break;
-    case ROW_LAST_PLAIN:
-        a.flags = 0;
-        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
-        break;
+    case ROW_FWD: IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
+    case ROW_FWD_NORM: IMR_LAUNCH((k_row<P, NL, RK_FWD_NORM, false, BL>), grid, block, lds, s, a); break;
+    case ROW_INV: IMR_LAUNCH((k_row<P, NL, RK_INV, false, BL>), grid, block, lds, s, a); break;
+    case ROW_INIT: IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL, 0, kInit>), grid, block, lds, s, a); break;
+    case ROW_LAST: IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL, 0, kLast>), grid, block, lds, s, a); break;
+    case ROW_LAST_PLAIN: IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL, 0, 0>), grid, block, lds, s, a); break;
     case ROW_INIT_ERR:
-        a.flags = kRowWriteG | kRowWriteBn | kRowError;
-        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL>), grid, block, lds, s, a);
+        IMR_LAUNCH((k_row<P, NL, RK_INVOUT, false, BL, 0, kInitErr>), grid, block, lds, s, a);
         break;
-    case ROW_FIRST_CUT:
-        a.flags = kRowCut;
-        IMR_LAUNCH((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
-        break;
-    case ROW_FIRST_MASK:
-        a.flags = 0;
-        IMR_LAUNCH((k_row<P, NL, RK_FIRST, false, BL>), grid, block, lds, s, a);
-        break;
-    case ROW_MID: a.flags = 0; IMR_LAUNCH((k_row<P, NL, RK_MID, PF, BL, STG>), grid, block, lds, s, a); break;
+    case ROW_FIRST_CUT: IMR_LAUNCH((k_row<P, NL, RK_FIRST, false, BL, 0, kRowCut>), grid, block, lds, s, a); break;
+    case ROW_FIRST_MASK: IMR_LAUNCH((k_row<P, NL, RK_FIRST, false, BL, 0, 0>), grid, block, lds, s, a); break;
+    case ROW_MID: IMR_LAUNCH((k_row<P, NL, RK_MID, PF, BL, STG>), grid, block, lds, s, a); break;
     default: return IMRESH_ERR_INVALID;
     }
     return from_hip(hipGetLastError());
