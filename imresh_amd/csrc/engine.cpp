@@ -64,7 +64,6 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     const size_t oPcnt = off; off = align256(off + 4 * (size_t) nrow);
     const size_t oScal = off; off = align256(off + 64);
     const size_t oFlags = off; off = align256(off + 64);
-    const size_t oAux = off; off = align256(off + sizeof(double) * kAuxDoubles);
     w->bytes = off;
     hipError_t e = hipMalloc(&w->base, off);
     if (e != hipSuccess) {
@@ -83,7 +82,6 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     w->pcnt = (unsigned *) (b + oPcnt);
     w->scal = (float *) (b + oScal);
     w->flags = (int *) (b + oFlags);
-    w->aux = (double *) (b + oAux);
     e = hipHostMalloc((void **) &w->host_stage, 4096, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&w->ready, hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -110,7 +108,7 @@ void ws_release(Workspace *w)
 size_t ws_bytes(int W, int H)
 {
     const size_t n = (size_t) W * H;
-    return 29 * n + 4 * (size_t) (blur_grid(W, H) + 2 * H) + 8 * 256 + sizeof(double) * kAuxDoubles + 256;
+    return 29 * n + 4 * (size_t) (blur_grid(W, H) + 2 * H) + 8 * 256;
 }
 
 }  // namespace imr
@@ -1166,14 +1164,10 @@ int imresh_poisson_modulus(float *modulus_dev, uint32_t width, uint32_t height, 
 {
     if (modulus_dev == nullptr) return IMRESH_ERR_NULL;
     if (width == 0 || height == 0 || !(photons > 0.0) || !(photons <= 1e30)) return IMRESH_ERR_INVALID;
-    int dev;
-    IMR_CHECK(current_device(&dev));
     hipStream_t s = S(stream);
-    Workspace *w;
-    IMR_CHECK(ws_checkout(dev, (int) width, (int) height, s, &w));
-    const int st = launch_poisson_modulus(modulus_dev, w->n, photons, seed, w->aux, s);
-    ws_return(w, s);
-    return st;
+    Scratch sc(s);   // the sums: kAuxDoubles doubles, stream-ordered
+    IMR_CHECK(sc.alloc(sizeof(double) * kAuxDoubles));
+    return launch_poisson_modulus(modulus_dev, (size_t) width * height, photons, seed, (double *) sc.p, s);
 }
 
 int imresh_fft_shift(void *data_dev, uint32_t nx, uint32_t ny, uint32_t element_bytes, void *stream)

@@ -34,7 +34,6 @@ struct Workspace {
     unsigned *pcnt = nullptr;
     float *scal = nullptr;      // [0] threshold, [1..3] error, total, nmasked
     int *flags = nullptr;       // [0] done, [1] cycles
-    double *aux = nullptr;      // kAuxDoubles doubles (measurement simulation sums)
     float *weights = nullptr;   // [cycles][kBlurWeightRow], zero-padded to radius wR
     unsigned wcap = 0;
     int wR = 0;

@@ -47,7 +47,6 @@ struct RowArgs {
     float *perr;           // per-workgroup sum mask*|g'|
     unsigned *pcnt;        // per-workgroup sum mask
     const int *done;       // early-exit flag (NULL: never)
-    int flags;             // unused by the row kernels (kept 0: the mode selects the kernel)
     int H;                 // number of rows (of one image)
     const float2 *tw;      // row-length twiddle table (set by launch_row)
     int batch;             // images (grid.y; 0 or 1: one image)
@@ -62,7 +61,6 @@ struct ColArgs {
     float2 *X;
     const float *M;        // modulus (COL_PROJ) or real input (COL_INIT)
     const int *done;
-    int flags;             // unused by the column kernels (kept 0)
     int W;                 // number of columns = row pitch
     const float2 *tw;      // column-length twiddle table (set by launch_col)
     int batch;             // images (grid.y; 0 or 1: one image)
@@ -114,8 +112,8 @@ struct BlurArgs {
 constexpr int kScalStride = 4;
 constexpr int kFlagStride = 2;
 constexpr int kBlurWeightRow = 64;
-// double scratch of a workspace: per-workgroup sums + total of the Poisson
-// measurement simulation
+// double scratch of the Poisson measurement simulation: per-workgroup sums
+// + total
 constexpr int kPoissonSumBlocks = 256;
 constexpr int kAuxDoubles = kPoissonSumBlocks + 1;
 

@@ -130,24 +130,19 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
     }
     switch (mode) {
     case COL_FWD:
-        a.flags = 0;
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD, false, BL>), grid, block, lds, s, a);
         break;
     case COL_REAL_FWD:
-        a.flags = 0;
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_FWD_REAL, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INV:
-        a.flags = 0;
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_INV, false, BL>), grid, block, lds, s, a);
         break;
     case COL_INIT:
-        a.flags = 0;
         a.out = a.X;
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_INV_REAL, false, BL>), grid, block, lds, s, a);
         break;
     case COL_PROJ:
-        a.flags = 0;
         IMR_LAUNCH((k_col<P, C, SPLIT, CK_PROJ, PF, BL, STG>), grid, block, lds, s, a);
         break;
     default: return IMRESH_ERR_INVALID;

@@ -11,7 +11,7 @@
 //             mask cut-off (cudaVectorElementwise.cu:198-219 with the device
 //             threshold) is fused into the load, then constraint + forward FFT.
 //   RK_INVOUT inverse row FFT closing a reconstruction step: writes g' and,
-//             by flags, gPrev = g' (cudaShrinkWrap.cu:304-309), |g'| for the
+//             by its output set, gPrev = g' (cudaShrinkWrap.cu:304-309), |g'| for the
 //             blur (:240) and the HIO error sums (cudaVectorReduce.cu:284-355).
 //   RK_FWD / RK_INV  plain row transforms of the 2-D FFT entry point;
 //            RK_FWD_NORM stores |z| of the forward one (the forward model).
@@ -185,7 +185,6 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
         }();
         if (attr != hipSuccess) return from_hip(attr);
     }
-    a.flags = 0;
     switch (mode) {
     case ROW_FWD: IMR_LAUNCH((k_row<P, NL, RK_FWD, false, BL>), grid, block, lds, s, a); break;
     case ROW_FWD_NORM: IMR_LAUNCH((k_row<P, NL, RK_FWD_NORM, false, BL>), grid, block, lds, s, a); break;

@@ -121,7 +121,7 @@ __device__ __forceinline__ T block_reduce(T x, T *red, Op op)
     return red[0];
 }
 
-// RowArgs.flags
+// outputs of the closing row pass / cut-off of the first (template FL of k_row)
 constexpr int kRowWriteG = 1;     // write gPrev = g'
 constexpr int kRowWriteBn = 2;    // write |g'| (blur input)
 constexpr int kRowError = 4;      // per-workgroup HIO error sums
