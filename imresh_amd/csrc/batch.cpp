@@ -391,13 +391,13 @@ extern "C" {
 int imresh_shrinkwrap_batch(const imresh_image *images, uint32_t count, const imresh_params *params,
                             void *stream, imresh_result *results)
 {
-    return imr::run_batch(images, count, params, (hipStream_t) stream, false, results);
+    return imr::guarded([&] { return imr::run_batch(images, count, params, (hipStream_t) stream, false, results); });
 }
 
 int imresh_shrinkwrap_batch_device(const imresh_image *images, uint32_t count,
                                    const imresh_params *params, void *stream, imresh_result *results)
 {
-    return imr::run_batch(images, count, params, (hipStream_t) stream, true, results);
+    return imr::guarded([&] { return imr::run_batch(images, count, params, (hipStream_t) stream, true, results); });
 }
 
 }  // extern "C"

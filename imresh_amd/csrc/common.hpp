@@ -2,6 +2,8 @@
 // library: status codes, launch descriptors and the line-FFT plan layout.
 #pragma once
 
+#include <new>
+
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -34,6 +36,22 @@ inline int from_hip(hipError_t e)
     if (e == hipErrorOutOfMemory) return IMRESH_ERR_OUT_OF_MEMORY;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return IMRESH_ERR_NO_DEVICE;
     return IMRESH_ERR_HIP;
+}
+}  // namespace imr
+
+namespace imr {
+// C-ABI entry points report statuses: a C++ exception (a host allocation or
+// thread creation that fails) must not cross the C boundary.
+template <class F>
+inline int guarded(F &&f) noexcept
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return IMRESH_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        return IMRESH_ERR_INTERNAL;
+    }
 }
 }  // namespace imr
 

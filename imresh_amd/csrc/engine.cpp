@@ -679,6 +679,7 @@ const char *imresh_strerror(int status)
     case IMRESH_ERR_NOT_INITIALIZED: return "task queue not initialized";
     case IMRESH_ERR_BUDGET: return "request exceeds the HBM budget";
     case IMRESH_ERR_IO: return "file unreadable, unwritable or of an unsupported format";
+    case IMRESH_ERR_INTERNAL: return "internal host-side failure";
     default: return "unknown status";
     }
 }
@@ -762,8 +763,8 @@ int imresh_stream_destroy(void *stream)
     return from_hip(hipStreamDestroy(S(stream)));
 }
 
-int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
-                             const imresh_params *params, void *stream, imresh_result *result)
+static int shrinkwrap_device_impl(float *io_dev, uint32_t width, uint32_t height,
+                           const imresh_params *params, void *stream, imresh_result *result)
 {
     if (io_dev == nullptr) return IMRESH_ERR_NULL;
     if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
@@ -783,8 +784,8 @@ int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
     return st;
 }
 
-int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
-                      const imresh_params *params, void *stream, imresh_result *result)
+static int shrinkwrap_host_impl(float *io_host, uint32_t width, uint32_t height,
+                         const imresh_params *params, void *stream, imresh_result *result)
 {
     if (io_host == nullptr) return IMRESH_ERR_NULL;
     if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
@@ -810,6 +811,18 @@ int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
     }
     ws_return(w, s);
     return st;
+}
+
+int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
+                             const imresh_params *params, void *stream, imresh_result *result)
+{
+    return guarded([&] { return shrinkwrap_device_impl(io_dev, width, height, params, stream, result); });
+}
+
+int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
+                      const imresh_params *params, void *stream, imresh_result *result)
+{
+    return guarded([&] { return shrinkwrap_host_impl(io_host, width, height, params, stream, result); });
 }
 
 }  // extern "C"
@@ -988,16 +1001,20 @@ int imresh_hybrid_input_output(float *io, const uint8_t *mask, uint32_t width, u
                                uint32_t n_cycles, float target_error, float hio_beta, void *stream,
                                imresh_result *result)
 {
-    return hio_standalone_entry(io, mask, width, height, n_cycles, target_error, hio_beta, stream,
-                                result, false);
+    return guarded([&] {
+        return hio_standalone_entry(io, mask, width, height, n_cycles, target_error, hio_beta, stream,
+                                    result, false);
+    });
 }
 
 int imresh_hybrid_input_output_device(float *io_dev, const uint8_t *mask_dev, uint32_t width,
                                       uint32_t height, uint32_t n_cycles, float target_error,
                                       float hio_beta, void *stream, imresh_result *result)
 {
-    return hio_standalone_entry(io_dev, mask_dev, width, height, n_cycles, target_error, hio_beta,
-                                stream, result, true);
+    return guarded([&] {
+        return hio_standalone_entry(io_dev, mask_dev, width, height, n_cycles, target_error, hio_beta,
+                                    stream, result, true);
+    });
 }
 
 int imresh_hio_iterations(float *gprime_dev, float *gprev_dev, const uint8_t *mask_dev,
@@ -1151,12 +1168,12 @@ int imresh_fft_supported(uint32_t n) { return line_supported((int) n) ? 1 : 0; }
 
 int imresh_diffraction_intensity(float *io, uint32_t width, uint32_t height, void *stream, int async)
 {
-    return diffraction_entry(io, width, height, stream, false, async != 0);
+    return guarded([&] { return diffraction_entry(io, width, height, stream, false, async != 0); });
 }
 
 int imresh_diffraction_intensity_device(float *io_dev, uint32_t width, uint32_t height, void *stream)
 {
-    return diffraction_entry(io_dev, width, height, stream, true, true);
+    return guarded([&] { return diffraction_entry(io_dev, width, height, stream, true, true); });
 }
 
 int imresh_poisson_modulus(float *modulus_dev, uint32_t width, uint32_t height, double photons,

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <new>
 #include <sstream>
 #include <string>
 #include <utility>
@@ -105,7 +106,9 @@ bool png_read(const char *path, std::vector<float> &out, uint32_t &W, uint32_t &
         }
         p += 12 + (size_t) len;
     }
-    if (!have_ihdr || w == 0 || h == 0 || interlace != 0) return false;
+    // sizes come from the file: at most 2^31 pixels (the library's sizes are
+    // 32-bit), so a corrupt header cannot request an absurd allocation
+    if (!have_ihdr || w == 0 || h == 0 || interlace != 0 || (uint64_t) w * h > (1ull << 31)) return false;
     int channels;
     switch (ctype) {
     case 0: channels = 1; break;   // gray
@@ -285,6 +288,7 @@ bool tiff_read(const char *path, std::vector<float> &out, uint32_t &W, uint32_t 
     // the layout the reference asserts: IEEE float samples (ReadTiff.cpp),
     // one per pixel, uncompressed strips
     if (w == 0 || h == 0 || bits != 32 || fmt != 3 || spp != 1 || comp != 1 || planar != 1) return false;
+    if ((uint64_t) w * h > (1ull << 31)) return false;
     if (offsets.empty() || offsets.size() != counts.size()) return false;
     const size_t need = (size_t) w * h * 4;
     std::vector<uint8_t> buf;
@@ -346,7 +350,13 @@ int read_entry(Reader rd, const char *path, float *out, size_t capacity, uint32_
     if (path == nullptr || width == nullptr || height == nullptr) return IMRESH_ERR_NULL;
     std::vector<float> img;
     uint32_t w = 0, h = 0;
-    if (!rd(path, img, w, h)) return IMRESH_ERR_IO;
+    try {
+        if (!rd(path, img, w, h)) return IMRESH_ERR_IO;
+    } catch (const std::bad_alloc &) {
+        return IMRESH_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        return IMRESH_ERR_IO;
+    }
     *width = w;
     *height = h;
     if (out == nullptr) return IMRESH_OK;   // size query
@@ -359,7 +369,11 @@ imresh::io::readInFuncs::FloatImage to_image(Reader rd, const std::string &path)
 {
     std::vector<float> img;
     uint32_t w = 0, h = 0;
-    if (!rd(path.c_str(), img, w, h)) return {nullptr, {0, 0}};
+    try {
+        if (!rd(path.c_str(), img, w, h)) return {nullptr, {0, 0}};
+    } catch (...) {
+        return {nullptr, {0, 0}};
+    }
     float *mem = new float[img.size()];
     std::memcpy(mem, img.data(), sizeof(float) * img.size());
     return {mem, {w, h}};
@@ -388,7 +402,11 @@ int imresh_write_png(const char *path, const float *data, uint32_t width, uint32
 {
     if (path == nullptr || data == nullptr) return IMRESH_ERR_NULL;
     if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
-    return png_write(path, data, width, height) ? IMRESH_OK : IMRESH_ERR_IO;
+    try {
+        return png_write(path, data, width, height) ? IMRESH_OK : IMRESH_ERR_IO;
+    } catch (const std::bad_alloc &) {
+        return IMRESH_ERR_OUT_OF_MEMORY;
+    }
 }
 
 }  // extern "C"
@@ -414,7 +432,11 @@ void writeOutPNG(float *rMem, unsigned int const rImageWidth, unsigned int const
                  std::string const rFileName)
 {
     if (rMem == nullptr || rImageWidth == 0 || rImageHeight == 0) return;
-    png_write(rFileName.c_str(), rMem, rImageWidth, rImageHeight);
+    try {
+        png_write(rFileName.c_str(), rMem, rImageWidth, rImageHeight);
+    } catch (const std::bad_alloc &) {
+        // the reference's pngwriter reports nothing either; the image is not written
+    }
 }
 
 void writeOutAndFreePNG(float *rMem, unsigned int const rImageWidth, unsigned int const rImageHeight,

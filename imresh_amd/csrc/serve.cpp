@@ -291,8 +291,7 @@ struct imresh_server {
     Server s;
 };
 
-int imresh_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t lanes,
-                         imresh_server **out)
+static int server_create_impl(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t lanes, imresh_server **out)
 {
     if (out == nullptr) return IMRESH_ERR_NULL;
     *out = nullptr;
@@ -313,14 +312,20 @@ int imresh_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_
         }
         s.lanes.push_back(st);
     }
-    s.thr = std::thread([&s] { s.loop(); });
+    try {
+        s.thr = std::thread([&s] { s.loop(); });
+    } catch (...) {
+        for (hipStream_t x : s.lanes) hipStreamDestroy(x);
+        delete h;
+        return IMRESH_ERR_INTERNAL;
+    }
     *out = h;
     return IMRESH_OK;
 }
 
-int imresh_server_submit(imresh_server *h, float *io, uint32_t width, uint32_t height,
-                         const imresh_params *params, int device_io, imresh_server_callback cb,
-                         void *user, uint64_t *id)
+static int server_submit_impl(imresh_server *h, float *io, uint32_t width, uint32_t height,
+                       const imresh_params *params, int device_io, imresh_server_callback cb, void *user,
+                       uint64_t *id)
 {
     if (h == nullptr || io == nullptr) return IMRESH_ERR_NULL;
     if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
@@ -350,6 +355,20 @@ int imresh_server_submit(imresh_server *h, float *io, uint32_t width, uint32_t h
     if (id) *id = r->id;
     s.cv.notify_one();
     return IMRESH_OK;
+}
+
+int imresh_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t lanes,
+                         imresh_server **out)
+{
+    return imr::guarded([&] { return server_create_impl(hbm_budget_bytes, max_active, lanes, out); });
+}
+
+int imresh_server_submit(imresh_server *h, float *io, uint32_t width, uint32_t height,
+                         const imresh_params *params, int device_io, imresh_server_callback cb,
+                         void *user, uint64_t *id)
+{
+    return imr::guarded(
+        [&] { return server_submit_impl(h, io, width, height, params, device_io, cb, user, id); });
 }
 
 int imresh_server_poll(imresh_server *h, imresh_server_completion *out, uint32_t capacity,

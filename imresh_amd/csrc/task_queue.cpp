@@ -136,7 +136,10 @@ int queue_add(Task &&t)
 
 extern "C" {
 
-int imresh_task_queue_init(int streams_per_device) { return queue_init(streams_per_device); }
+int imresh_task_queue_init(int streams_per_device)
+{
+    return imr::guarded([&] { return queue_init(streams_per_device); });
+}
 
 int imresh_task_queue_deinit(void) { return queue_deinit(); }
 
@@ -154,9 +157,11 @@ int imresh_add_task(float *io_host, uint32_t width, uint32_t height, const imres
         imresh_params_default(&t.params);
         t.params.defaults = IMRESH_DEFAULTS_QUEUE;
     }
-    if (callback)
-        t.done = [callback, user](float *io, uint32_t w, uint32_t h, int st) { callback(user, io, w, h, st); };
-    return queue_add(std::move(t));
+    return imr::guarded([&] {
+        if (callback)
+            t.done = [callback, user](float *io, uint32_t w, uint32_t h, int st) { callback(user, io, w, h, st); };
+        return queue_add(std::move(t));
+    });
 }
 
 }  // extern "C"

@@ -37,7 +37,8 @@ enum imresh_status {
     IMRESH_ERR_KERNEL_TOO_LARGE = 7, /* Gaussian kernel > 63 taps (gaussian.cpp:189-192) */
     IMRESH_ERR_NOT_INITIALIZED = 8,  /* addTask before taskQueueInit (taskQueue.cu:163) */
     IMRESH_ERR_BUDGET = 9,           /* request exceeds the configured HBM budget */
-    IMRESH_ERR_IO = 10               /* file unreadable / unwritable or format unsupported */
+    IMRESH_ERR_IO = 10,              /* file unreadable / unwritable or format unsupported */
+    IMRESH_ERR_INTERNAL = 11         /* unexpected host-side failure (e.g. thread creation) */
 };
 
 /* Default-substitution rule for parameters <= 0 (or == 0 for counts). */
