@@ -141,8 +141,17 @@ void class_return(ClassBuf *c, hipStream_t s)
     g_free[{c->device, c->W, c->H}].push_back(c);
 }
 
-// per-device internal stream (graph capture needs a non-legacy stream)
+// per-device internal stream (graph capture needs a non-legacy stream) and
+// the lock that gives one batch call at a time the use of it: a batch fills
+// the device, and two captures on one stream would interfere
 std::map<int, hipStream_t> g_streams;
+std::map<int, std::mutex> g_run_mtx;
+
+std::mutex &run_lock(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mtx);
+    return g_run_mtx[device];
+}
 
 int work_stream(int device, hipStream_t *out)
 {
@@ -238,6 +247,7 @@ int run_batch(const imresh_image *imgs, uint32_t count, const imresh_params *par
     int R = 0;
     IMR_CHECK(padded_weights(q, wrows, &R));
 
+    std::lock_guard<std::mutex> run(run_lock(dev));
     hipStream_t s;
     IMR_CHECK(work_stream(dev, &s));
     hipEvent_t ev = nullptr;

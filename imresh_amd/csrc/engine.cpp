@@ -19,6 +19,7 @@
 // The early exit sets a device flag that turns every later kernel of the
 // reconstruction into a no-op, so the host never waits inside the loop.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -478,7 +479,7 @@ int cycle_graph(Workspace *w, const Params &q, const Plan2D &p)
 
 }  // namespace
 
-bool g_use_graphs = true;
+std::atomic<bool> g_use_graphs{true};
 
 int recon_cycle(Workspace *w, const float *M, const Params &q, unsigned c, hipStream_t s)
 {

@@ -2,6 +2,8 @@
 // batched driver (batch.cpp) and the request server (serve.cpp).
 #pragma once
 
+#include <atomic>
+
 #include <vector>
 
 #include "internal.hpp"
@@ -60,7 +62,7 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s);
 // common radius
 int padded_weights(const Params &q, std::vector<float> &rows, int *R);
 // cycles run as captured graphs (default) or as eager launches
-extern bool g_use_graphs;
+extern std::atomic<bool> g_use_graphs;
 
 // one reconstruction in steps (all asynchronous on s): begin = weights,
 // modulus M (device) staged into the workspace, state reset, initial

@@ -167,3 +167,35 @@ def test_cycle_graphs_equal_eager_launches():
             y, s = _single(M, **kw)
             assert np.array_equal(x, y)
             assert r.last_error == s.last_error and r.cycles_run == s.cycles_run
+
+
+def test_concurrent_batch_calls_from_two_threads():
+    """Batch calls from several host threads on one device run one at a time
+    on the library's capture stream; each call's images equal their single
+    reconstructions."""
+    import threading
+    sets = [[(64, 64), (96, 96), (128, 128)], [(200, 200), (64, 64), (97, 131)],
+            [(128, 128), (256, 256)]]
+    moduli = [_moduli(sz, seed=40 + i) for i, sz in enumerate(sets)]
+    ios = [[M.copy() for M in Ms] for Ms in moduli]
+    errors = []
+
+    def work(k):
+        try:
+            for _ in range(3):
+                for io, M in zip(ios[k], moduli[k]):
+                    io[...] = M
+                im.shrink_wrap_batch(ios[k], _params(**KW))
+        except Exception as e:   # reported below, in the test's thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(sets))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=240)
+    assert not errors, errors
+    assert not any(t.is_alive() for t in threads)
+    for Ms, outs in zip(moduli, ios):
+        for M, io in zip(Ms, outs):
+            assert np.array_equal(io, _single(M, **KW)[0]), M.shape
