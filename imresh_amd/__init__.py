@@ -132,6 +132,8 @@ _SIGS = {
     "imresh_poisson_modulus": (c_int, [_P, c_uint32, c_uint32, c_double, c_uint64, _P]),
     "imresh_fft_shift": (c_int, [_P, c_uint32, c_uint32, c_uint32, _P]),
     "imresh_autotune": (c_int, [c_uint32, c_uint32, _P, POINTER(c_int), POINTER(c_int)]),
+    "imresh_release_cached": (c_int, []),
+    "imresh_cached_bytes": (c_int, [POINTER(c_uint64)]),
     "imresh_variant_counts": (c_int, [c_uint32, c_uint32, POINTER(c_int), POINTER(c_int)]),
     "imresh_set_variants": (c_int, [c_uint32, c_uint32, c_int, c_int]),
     "imresh_task_queue_init": (c_int, [c_int]),

@@ -50,10 +50,39 @@ struct ClassBuf {
     hipStream_t branch = nullptr;
     hipEvent_t join = nullptr, ready = nullptr;
     bool pending = false;
+    size_t bytes = 0;        // device bytes of the allocation
+    uint64_t last_use = 0;   // pool clock of the last return
 };
 
 std::mutex g_mtx;
 std::map<std::tuple<int, int, int>, std::vector<ClassBuf *>> g_free;
+size_t g_idle = 0;       // bytes of the idle buffers in g_free
+uint64_t g_clock = 0;
+
+// under g_mtx: detach idle buffers whose work has finished, least recently
+// returned first, until the idle bytes fit `cap`
+void trim_classes(size_t cap, std::vector<ClassBuf *> &out)
+{
+    while (g_idle > cap) {
+        ClassBuf *lru = nullptr;
+        std::vector<ClassBuf *> *list = nullptr;
+        size_t at = 0;
+        for (auto &kv : g_free)
+            for (size_t i = 0; i < kv.second.size(); ++i) {
+                ClassBuf *c = kv.second[i];
+                if (c->pending && hipEventQuery(c->ready) != hipSuccess) continue;
+                if (lru == nullptr || c->last_use < lru->last_use) {
+                    lru = c;
+                    list = &kv.second;
+                    at = i;
+                }
+            }
+        if (lru == nullptr) return;
+        list->erase(list->begin() + (long) at);
+        g_idle -= lru->bytes;
+        out.push_back(lru);
+    }
+}
 
 void class_free(ClassBuf *c)
 {
@@ -88,6 +117,7 @@ int class_alloc(int device, int W, int H, int cap, ClassBuf **out)
     const size_t oScal = off; off = align256(off + sizeof(float) * kScalStride * cap);
     const size_t oFlags = off; off = align256(off + sizeof(int) * kFlagStride * cap);
     const size_t oTab = off; off = align256(off + sizeof(void *) * cap);
+    c->bytes = off;
     hipError_t e = hipMalloc(&c->base, off);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->branch, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
@@ -124,6 +154,7 @@ int class_checkout(int device, int W, int H, int count, hipStream_t s, ClassBuf 
             if (v[i]->cap >= count) {
                 c = v[i];
                 v.erase(v.begin() + (long) i);
+                g_idle -= c->bytes;
                 break;
             }
     }
@@ -137,8 +168,15 @@ void class_return(ClassBuf *c, hipStream_t s)
 {
     c->pending = hipEventRecord(c->ready, s) == hipSuccess;
     if (!c->pending) hipStreamSynchronize(s);
-    std::lock_guard<std::mutex> lk(g_mtx);
-    g_free[{c->device, c->W, c->H}].push_back(c);
+    std::vector<ClassBuf *> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_mtx);
+        c->last_use = ++g_clock;
+        g_free[{c->device, c->W, c->H}].push_back(c);
+        g_idle += c->bytes;
+        trim_classes(pool_cap(), drop);
+    }
+    for (ClassBuf *x : drop) class_free(x);
 }
 
 // per-device internal stream (graph capture needs a non-legacy stream) and
@@ -393,6 +431,27 @@ int run_batch(const imresh_image *imgs, uint32_t count, const imresh_params *par
 }
 
 }  // namespace
+
+}  // namespace imr
+
+namespace imr {
+
+int trim_idle_classes(size_t cap)
+{
+    std::vector<ClassBuf *> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_mtx);
+        trim_classes(cap, drop);
+    }
+    for (ClassBuf *x : drop) class_free(x);
+    return IMRESH_OK;
+}
+
+size_t idle_class_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_mtx);
+    return g_idle;
+}
 
 }  // namespace imr
 

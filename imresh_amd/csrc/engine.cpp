@@ -128,6 +128,40 @@ std::map<std::tuple<int, int, int>, std::vector<Workspace *>> g_ws_free;
 constexpr int kMaxWorkspaces = 4;
 std::map<std::tuple<int, int, int>, int> g_ws_count;
 
+// Idle workspaces stay pooled for the next reconstruction of their size, up
+// to g_pool_cap bytes in all (IMRESH_OPT_POOL_MIB); beyond it the least
+// recently returned idle ones whose work has finished are freed, so a stream
+// of many distinct sizes does not keep one workspace per size forever.
+size_t g_idle_bytes = 0;
+std::atomic<size_t> g_pool_cap{(size_t) 32 << 30};   // read without g_ws_mtx (batch.cpp)
+uint64_t g_use_clock = 0;
+
+// under g_ws_mtx: detach idle workspaces (work finished) until the idle bytes
+// fit `cap`, least recently returned first; the caller frees them unlocked
+void trim_idle(size_t cap, std::vector<Workspace *> &out)
+{
+    while (g_idle_bytes > cap) {
+        Workspace *lru = nullptr;
+        std::vector<Workspace *> *list = nullptr;
+        size_t at = 0;
+        for (auto &kv : g_ws_free)
+            for (size_t i = 0; i < kv.second.size(); ++i) {
+                Workspace *w = kv.second[i];
+                if (w->pending && hipEventQuery(w->ready) != hipSuccess) continue;
+                if (lru == nullptr || w->last_use < lru->last_use) {
+                    lru = w;
+                    list = &kv.second;
+                    at = i;
+                }
+            }
+        if (lru == nullptr) return;   // everything idle is still running
+        list->erase(list->begin() + (long) at);
+        --g_ws_count[{lru->device, lru->W, lru->H}];
+        g_idle_bytes -= lru->bytes;
+        out.push_back(lru);
+    }
+}
+
 int ws_checkout(int device, int W, int H, hipStream_t s, Workspace **out)
 {
     Workspace *w = nullptr;
@@ -148,6 +182,7 @@ int ws_checkout(int device, int W, int H, hipStream_t s, Workspace **out)
                 ++count;
             }
         }
+        if (w != nullptr) g_idle_bytes -= w->bytes;
     }
     if (w == nullptr) {
         const int st = ws_alloc(device, W, H, out);
@@ -166,13 +201,22 @@ void ws_return(Workspace *w, hipStream_t s)
 {
     w->pending = hipEventRecord(w->ready, s) == hipSuccess;
     if (!w->pending) hipStreamSynchronize(s);
-    std::lock_guard<std::mutex> lk(g_ws_mtx);
-    g_ws_free[{w->device, w->W, w->H}].push_back(w);
+    std::vector<Workspace *> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mtx);
+        w->last_use = ++g_use_clock;
+        g_ws_free[{w->device, w->W, w->H}].push_back(w);
+        g_idle_bytes += w->bytes;
+        trim_idle(g_pool_cap, drop);
+    }
+    for (Workspace *x : drop) ws_release(x);
 }
 
 }  // namespace
 
 namespace imr {
+
+size_t pool_cap() { return g_pool_cap.load(); }
 
 int normalize(const imresh_params *p, Params &q)
 {
@@ -745,8 +789,42 @@ int imresh_set_option(int option, int value)
 {
     switch (option) {
     case IMRESH_OPT_CYCLE_GRAPHS: g_use_graphs = value != 0; return IMRESH_OK;
+    case IMRESH_OPT_POOL_MIB: {
+        if (value < 0) return IMRESH_ERR_INVALID;
+        std::vector<Workspace *> drop;
+        {
+            std::lock_guard<std::mutex> lk(g_ws_mtx);
+            g_pool_cap = (size_t) value << 20;
+            trim_idle(g_pool_cap, drop);
+        }
+        for (Workspace *x : drop) ws_release(x);
+        return guarded([&] { return trim_idle_classes((size_t) value << 20); });
+    }
     default: return IMRESH_ERR_INVALID;
     }
+}
+
+int imresh_release_cached(void)
+{
+    std::vector<Workspace *> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mtx);
+        trim_idle(0, drop);
+    }
+    for (Workspace *x : drop) ws_release(x);
+    return guarded([] { return trim_idle_classes(0); });
+}
+
+int imresh_cached_bytes(uint64_t *bytes)
+{
+    if (bytes == nullptr) return IMRESH_ERR_NULL;
+    size_t ws = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mtx);
+        ws = g_idle_bytes;
+    }
+    *bytes = ws + idle_class_bytes();   // the two pools' locks are never held together
+    return IMRESH_OK;
 }
 
 int imresh_stream_create(void **stream)

@@ -45,6 +45,7 @@ struct Workspace {
     float *host_stage = nullptr;  // pinned, 64 floats + readback
     hipEvent_t ready = nullptr;   // recorded when the last user's work is queued
     bool pending = false;
+    uint64_t last_use = 0;        // pool clock of the last return (idle trimming)
     // the captured cycle graph and the parameters it was captured with
     hipGraphExec_t gexec = nullptr;
     unsigned g_nhio = 0;
@@ -54,6 +55,12 @@ struct Workspace {
 };
 
 int ws_alloc(int device, int W, int H, Workspace **out);
+// batch class buffers: free idle ones whose work has finished, least
+// recently used first, until the idle bytes fit `cap`; bytes held idle
+int trim_idle_classes(size_t cap);
+size_t idle_class_bytes();
+// bytes that idle pooled buffers of one kind (workspaces, batch buffers) may keep
+size_t pool_cap();
 void ws_release(Workspace *w);
 // device bytes of a workspace (for HBM budgets)
 size_t ws_bytes(int W, int H);

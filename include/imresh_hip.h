@@ -91,9 +91,16 @@ int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream);
 int imresh_stream_sync(void *stream);
 /* Library options (process-wide).  IMRESH_OPT_CYCLE_GRAPHS: 1 (default)
  * replays each shrink-wrap cycle as a captured HIP graph, 0 launches its
- * kernels one by one; both compute the same bits. */
-enum { IMRESH_OPT_CYCLE_GRAPHS = 1 };
+ * kernels one by one; both compute the same bits.  IMRESH_OPT_POOL_MIB: the
+ * device memory (MiB, default 32768) that idle per-size workspaces may keep
+ * between calls; beyond it the least recently used idle ones are freed. */
+enum { IMRESH_OPT_CYCLE_GRAPHS = 1, IMRESH_OPT_POOL_MIB = 2 };
 int imresh_set_option(int option, int value);
+/* Frees every idle pooled workspace and batch buffer whose work has
+ * finished (all of them after the caller's streams are synchronized). */
+int imresh_release_cached(void);
+/* Device bytes held by idle pooled workspaces and batch buffers. */
+int imresh_cached_bytes(uint64_t *bytes);
 
 /* A non-blocking HIP stream at the default priority (for callers without a
  * HIP runtime of their own, e.g. to overlap reconstructions), and its

@@ -463,3 +463,50 @@ def test_hio_error_bit_packed_pythagorean():
     im.check(L().imresh_calculate_hio_error_bits(dd.ptr, db.ptr, 1000, 1, ctypes.byref(err),
                                                  ctypes.byref(tot), ctypes.byref(cnt), None))
     assert cnt.value == 1000 - mask[:1000].sum()
+
+
+# ---------------------------------------------------------------- pooled buffers
+def test_pool_cap_and_release():
+    """Idle per-size workspaces are kept up to IMRESH_OPT_POOL_MIB and freed
+    least recently used first beyond it; imresh_release_cached frees every
+    idle workspace and batch buffer.  Results do not depend on pooling."""
+    cached = ctypes.c_uint64()
+    p = im.default_params(n_cycles=2, n_hio_cycles=3)
+    Ms = [syn.workload(n, n, n_shapes=4, central=n // 2, index=n)[1] for n in (192, 256, 320)]
+
+    def run_all():
+        outs = []
+        for M in Ms:
+            x = M.copy()
+            im.check(L().imresh_shrinkwrap(x.ctypes.data, x.shape[1], x.shape[0], p, None, None))
+            outs.append(x)
+        return outs
+
+    ref = run_all()
+    try:
+        im.check(L().imresh_set_option(2, 1))          # IMRESH_OPT_POOL_MIB: 1 MiB
+        im.check(L().imresh_cached_bytes(ctypes.byref(cached)))
+        assert cached.value <= 1 << 20
+        again = run_all()
+        # the workspace returned last may still count as busy (its completion
+        # event was just recorded); the next trim, here re-applying the cap,
+        # frees it
+        im.check(L().imresh_stream_sync(None))
+        im.check(L().imresh_set_option(2, 1))
+        im.check(L().imresh_cached_bytes(ctypes.byref(cached)))
+        assert cached.value <= 1 << 20
+        for a, b in zip(ref, again):
+            assert np.array_equal(a, b)
+        im.check(L().imresh_set_option(2, 32768))
+        run_all()
+        im.shrink_wrap_batch([M.copy() for M in Ms], p)
+        im.check(L().imresh_cached_bytes(ctypes.byref(cached)))
+        assert cached.value > 1 << 20
+        im.check(L().imresh_release_cached())
+        im.check(L().imresh_cached_bytes(ctypes.byref(cached)))
+        assert cached.value == 0
+        for a, b in zip(ref, run_all()):
+            assert np.array_equal(a, b)
+        assert L().imresh_set_option(2, -1) == 2
+    finally:
+        im.check(L().imresh_set_option(2, 32768))
