@@ -72,7 +72,8 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def cpu_baseline(M: np.ndarray, M256: np.ndarray, budget_s: float = 12.0, max_iter: int = 20) -> dict:
+def cpu_baseline(M: np.ndarray, M256: np.ndarray, M8=None, budget_s: float = 12.0,
+                 max_iter: int = 20) -> dict:
     import oracle  # CPU checker; only the baseline leg uses it
 
     oracle.set_fft_threads(1)   # the reference plans FFTW single-threaded
@@ -89,14 +90,70 @@ def cpu_baseline(M: np.ndarray, M256: np.ndarray, budget_s: float = 12.0, max_it
     t1 = time.perf_counter()
     oracle.shrink_wrap(M256, n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
     dt256 = time.perf_counter() - t1
-    return {"value": round(done / dt, 4), "unit": "HIO iters/s", "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": f"{done} HIO iterations (constraint, FFT, modulus, IFFT) of the 2048x2048 "
-                      f"workload on the CPU oracle; FFT single-threaded as the reference, "
-                      f"elementwise loops on {oracle.num_threads()} OpenMP threads; "
-                      f"{dt:.1f} s",
-            "recon_256_s": round(dt256, 3),
-            "recon_256_note": "one full 20x20 reconstruction of the 256x256 workload on the CPU oracle"}
+    out = {"value": round(done / dt, 4), "unit": "HIO iters/s", "cores": oracle.num_threads(),
+           "kind": "port",
+           "sample": f"{done} HIO iterations (constraint, FFT, modulus, IFFT) of the 2048x2048 "
+                     f"workload on the CPU oracle; FFT single-threaded as the reference, "
+                     f"elementwise loops on {oracle.num_threads()} OpenMP threads; "
+                     f"{dt:.1f} s",
+           "recon_256_s": round(dt256, 3),
+           "recon_256_note": "one full 20x20 reconstruction of the 256x256 workload on the CPU oracle",
+           "per_op_ms": cpu_per_op(g, gp, mask, M)}
+    if M8 is not None:
+        out["large_8192"] = cpu_large(M8)
+    return out
+
+
+def _time_ms(fn) -> float:
+    t = time.perf_counter()
+    fn()
+    return round((time.perf_counter() - t) * 1e3, 2)
+
+
+def cpu_per_op(g, gp, mask, M) -> dict:
+    """Each operator of one HIO iteration and of one cycle's mask update,
+    timed once on the CPU oracle at the workload's size (the reference's CPU
+    operators, shrinkWrap.cpp:235-301), for comparison with the GPU's fused
+    kernels: the FFTs single-threaded, the rest on the OpenMP threads."""
+    import oracle
+    absg = oracle.complex_norm(g)
+    thr = 0.2 * oracle.vmax(absg)
+    return {
+        "fft2d_forward": _time_ms(lambda: oracle.fft2d(gp, -1)),
+        "fft2d_inverse": _time_ms(lambda: oracle.fft2d(g, +1)),
+        "hio_constraint": _time_ms(lambda: oracle.hio_constraint(gp, g, mask, 0.9)),
+        "apply_modulus": _time_ms(lambda: oracle.apply_modulus(g, M)),
+        "complex_norm": _time_ms(lambda: oracle.complex_norm(g)),
+        "gaussian_blur_sigma3": _time_ms(lambda: oracle.blur(absg, 3.0)),
+        "vector_max": _time_ms(lambda: oracle.vmax(absg)),
+        "cutoff": _time_ms(lambda: oracle.cutoff(absg, thr)),
+        "hio_error": _time_ms(lambda: oracle.hio_error(g, mask)),
+    }
+
+
+def cpu_large(M8: np.ndarray) -> dict:
+    """The 8192 x 8192 workload on the CPU oracle: one HIO iteration and one
+    cycle's mask update timed, a full 20 x 20 reconstruction extrapolated
+    (the initial inverse FFT + 20 mask updates + 400 iterations); one
+    iteration is two single-threaded 8192^2 FFTs, tens of seconds."""
+    import oracle
+    oracle.set_fft_threads(1)
+    t = time.perf_counter()
+    g = oracle.fft2d(M8.astype(np.complex64), +1, np.float32)
+    t_ifft = time.perf_counter() - t
+    t = time.perf_counter()
+    mask = oracle.make_mask(g, 6.0, 0.04, np.float32)
+    t_mask = time.perf_counter() - t
+    gp = g.copy()
+    t = time.perf_counter()
+    oracle.hio_iterations(g, gp, mask, M8, 0.9, 1, np.float32)
+    t_iter = time.perf_counter() - t
+    total = t_ifft + N_CYCLES * t_mask + N_CYCLES * N_HIO * t_iter
+    return {"hio_iter_s": round(t_iter, 3), "mask_update_s": round(t_mask, 3),
+            "initial_ifft_s": round(t_ifft, 3), "recon_s_extrapolated": round(total, 1),
+            "hio_iters_per_s": round(1.0 / t_iter, 5),
+            "note": "1 HIO iteration and 1 mask update (sigma 6, 35 taps) timed on the CPU oracle, "
+                    "20x20 reconstruction extrapolated linearly"}
 
 
 def _gen_batch_input(args):
@@ -326,6 +383,58 @@ def large_bench(L, M8, steps: int = 2) -> dict:
             "iteration_alg_GBps_80B": round(80 * n * n * N_CYCLES * N_HIO / dt / 1e9, 1)}
 
 
+def gpu_per_op(L, M: np.ndarray, reps: int = 20) -> dict:
+    """The per-operator entry points (the reference's unfused operator
+    sequence, one launch per operator) at the workload's size, ms per call
+    from host timing of `reps` back-to-back calls on the default stream;
+    the reductions return their scalar to the host and synchronize per call
+    as the reference's do (cudaVectorReduce.cu:198-211,379-395).  The state
+    is the workload's: g' = IFFT2(M), the mask from one cycle's update."""
+    h, w = M.shape
+    n = w * h
+    c = im.ctypes
+    dM = im.DeviceBuffer.from_array(M)
+    g, gp, tmp = im.DeviceBuffer(8 * n), im.DeviceBuffer(8 * n), im.DeviceBuffer(8 * n)
+    a, mask = im.DeviceBuffer(4 * n), im.DeviceBuffer(4 * n)
+    mx, err = c.c_float(), c.c_float()
+    im.check(L.imresh_copy_to_real_part(g.ptr, dM.ptr, n, None), "copy")
+    im.check(L.imresh_fft2d(g.ptr, g.ptr, w, h, +1, None), "ifft")
+    im.check(L.imresh_memcpy_d2d(gp.ptr, g.ptr, 8 * n, None), "d2d")
+    im.check(L.imresh_complex_norm(a.ptr, g.ptr, n, None), "norm")
+    im.check(L.imresh_gaussian_blur(a.ptr, w, h, 3.0, None), "blur")
+    im.check(L.imresh_vector_max(a.ptr, n, c.byref(mx), None), "max")
+    im.check(L.imresh_cutoff(a.ptr, n, np.float32(0.04) * np.float32(mx.value), 1.0, 0.0, None), "cut")
+    im.check(L.imresh_memcpy_d2d(mask.ptr, a.ptr, 4 * n, None), "d2d")
+    im.sync()
+    ops = {
+        "fft2d_forward": lambda: L.imresh_fft2d(tmp.ptr, gp.ptr, w, h, -1, None),
+        "fft2d_inverse": lambda: L.imresh_fft2d(tmp.ptr, g.ptr, w, h, +1, None),
+        "hio_constraint": lambda: L.imresh_hio_domain_constraints(gp.ptr, g.ptr, mask.ptr, n,
+                                                                  np.float32(0.9), None),
+        "apply_modulus": lambda: L.imresh_apply_complex_modulus(tmp.ptr, g.ptr, dM.ptr, n, None),
+        "complex_norm": lambda: L.imresh_complex_norm(a.ptr, g.ptr, n, None),
+        "gaussian_blur_sigma3": lambda: L.imresh_gaussian_blur(a.ptr, w, h, 3.0, None),
+        "vector_max": lambda: L.imresh_vector_max(a.ptr, n, c.byref(mx), None),
+        "cutoff": lambda: L.imresh_cutoff(a.ptr, n, np.float32(mx.value), 1.0, 0.0, None),
+        "hio_error": lambda: L.imresh_calculate_hio_error(g.ptr, mask.ptr, n, 0, c.byref(err),
+                                                          None, None, None),
+    }
+    out = {}
+    for name, fn in ops.items():
+        im.check(fn(), name)
+        im.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            im.check(fn(), name)
+        im.sync()
+        out[name] = round((time.perf_counter() - t0) * 1e3 / reps, 4)
+    out["unfused_iteration"] = round(sum(out[k] for k in ("hio_constraint", "fft2d_forward",
+                                                          "apply_modulus", "fft2d_inverse")), 4)
+    for b in (dM, g, gp, tmp, a, mask):
+        b.close()
+    return out
+
+
 def small_bench(L, M256: np.ndarray, reps: int = 20) -> dict:
     """The reference's CPU-runnable case (256 x 256, 8 shapes, no noise,
     20 x 20) on the GPU: reconstructions/s of back-to-back calls with the
@@ -413,6 +522,9 @@ def main() -> int:
     ap.add_argument("--no-fft-compare", action="store_true")
     ap.add_argument("--no-forward-model", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--no-per-op", action="store_true")
+    ap.add_argument("--cpu-large", action="store_true",
+                    help="also time the 8192x8192 workload on the CPU oracle (about a minute)")
     ap.add_argument("--batch-count", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -424,7 +536,7 @@ def main() -> int:
     obj, M = syn.config_2048(0)
     M256 = syn.config_256(0)[1]
     M8 = None
-    if not args.no_large:
+    if not args.no_large or args.cpu_large:
         log("[bench] generating the 8192x8192 workload")
         M8 = syn.config_8192(0)[1]
     (bsizes, bmods), (trace, smods), (light, lmods) = generate_inputs(
@@ -507,10 +619,17 @@ def main() -> int:
         serve["below_capacity"] = serve_bench(L, light, lmods, rate=50.0, modes=(0,))
 
     large = None
-    if M8 is not None:
+    if M8 is not None and not args.no_large:
         log("[bench] 8192x8192 reconstruction")
         large = large_bench(L, M8)
-        del M8
+        if not args.cpu_large:
+            M8 = None
+
+    per_op = None
+    if not args.no_per_op:
+        log("[bench] per-operator entry points (unfused sequence)")
+        per_op = gpu_per_op(L, M)
+        per_op["fused_iteration"] = round(1e3 / value, 4)
 
     small = None
     if not args.no_small:
@@ -531,7 +650,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
-        cpu = cpu_baseline(M, M256)
+        cpu = cpu_baseline(M, M256, M8 if args.cpu_large else None)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "HIO iters/s", "n_gpus": 1,
@@ -544,7 +663,8 @@ def main() -> int:
                    "row_variant": rv.value, "col_variant": cv.value},
         "roofline": roofline, "cpu_baseline": cpu, "quality": quality, "small_256": small,
         "batched": batch,
-        "serving": serve, "large_8192": large, "fft_vs_rocfft": fftc, "forward_model": fwd,
+        "serving": serve, "large_8192": large, "gpu_per_op_ms": per_op, "fft_vs_rocfft": fftc,
+        "forward_model": fwd,
     }
     print(json.dumps(line), flush=True)
     return 0

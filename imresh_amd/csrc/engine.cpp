@@ -645,24 +645,76 @@ int hio_error(const float *g, const void *mask, int kind, uint32_t n, int invert
     return IMRESH_OK;
 }
 
+// Per-device table of Gaussian weight vectors for the per-operator blur,
+// keyed by sigma, as the reference keeps up to 128 per device keyed by sigma
+// (GaussianKernelGpuBuffer, cudaGaussian.cu:64-298).  A slot holds the taps
+// at [0, k), 1.0 at [64] (the weight of an identity pass) and zeros
+// elsewhere; it is written once, on first use of its sigma (upload and
+// stream synchronization, as the reference's first use), and never changed,
+// so kernels on any stream read it while other sigmas are added.  With every
+// slot taken the caller stages its weights per call instead.
+constexpr int kWeightSlots = 128, kWeightSlotFloats = 128;
+struct WeightTable {
+    float *dev = nullptr;
+    std::map<uint64_t, int> slot;
+};
+std::mutex g_wt_mtx;
+std::map<int, WeightTable> g_wt;
+
+int cached_weights(int dev, double sigma, const float *host, hipStream_t s, const float **out)
+{
+    std::lock_guard<std::mutex> lk(g_wt_mtx);
+    WeightTable &t = g_wt[dev];
+    uint64_t key;
+    std::memcpy(&key, &sigma, sizeof key);
+    auto it = t.slot.find(key);
+    if (it != t.slot.end()) {
+        *out = t.dev + (size_t) it->second * kWeightSlotFloats;
+        return IMRESH_OK;
+    }
+    *out = nullptr;
+    if ((int) t.slot.size() >= kWeightSlots) return IMRESH_OK;
+    if (t.dev == nullptr)
+        IMR_HIP_CHECK(hipMalloc(&t.dev, sizeof(float) * kWeightSlots * kWeightSlotFloats));
+    const int idx = (int) t.slot.size();
+    float *p = t.dev + (size_t) idx * kWeightSlotFloats;
+    IMR_HIP_CHECK(hipMemcpyAsync(p, host, sizeof(float) * kWeightSlotFloats, hipMemcpyHostToDevice, s));
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    t.slot.emplace(key, idx);
+    *out = p;
+    return IMRESH_OK;
+}
+
+// The per-operator blur: asynchronous once sigma's weights are in the
+// device table (synchronous on first use of a sigma, and on every call
+// when the table is full, for the host weights must outlive their copy).
 int blur_impl(float *d, uint32_t W, uint32_t H, double sigma, int which, void *stream)
 {
     if (d == nullptr) return IMRESH_ERR_NULL;
     if (W == 0 || H == 0) return IMRESH_ERR_INVALID;
     const int k = gaussian_weights(sigma, nullptr, 0);
     if (k > 63) return IMRESH_ERR_KERNEL_TOO_LARGE;
-    float w[64 + 1];
+    float w[kWeightSlotFloats] = {};
     gaussian_weights(sigma, w, 64);
     w[64] = 1.f;
     hipStream_t s = S(stream);
-    Scratch sc(s);
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    const float *dw = nullptr;
+    IMR_CHECK(cached_weights(dev, sigma, w, s, &dw));
     const size_t n = (size_t) W * H;
-    IMR_CHECK(sc.alloc(sizeof(float) * (n + 128)));
-    float *dw = (float *) sc.p;
-    float *tmp = dw + 128;
-    IMR_HIP_CHECK(hipMemcpyAsync(dw, w, sizeof(w), hipMemcpyHostToDevice, s));
+    const bool h_inplace = which == 1 && (size_t) W * 4 + 256 <= 120 * 1024;
+    const size_t stage = dw == nullptr ? kWeightSlotFloats : 0;
+    Scratch sc(s);
+    const size_t need = stage + (h_inplace ? 0 : n);
+    if (need) IMR_CHECK(sc.alloc(sizeof(float) * need));
+    float *tmp = (float *) sc.p + stage;
+    if (dw == nullptr) {
+        IMR_HIP_CHECK(hipMemcpyAsync(sc.p, w, sizeof(w), hipMemcpyHostToDevice, s));
+        dw = (const float *) sc.p;
+    }
     const int r = (k - 1) / 2;
-    if (which == 1 && (size_t) W * 4 + 256 <= 120 * 1024) {
+    if (h_inplace) {
         IMR_CHECK(launch_blur_h_inplace(d, W, H, dw, r, s));
     } else {
         BlurArgs ba{};
@@ -679,8 +731,8 @@ int blur_impl(float *d, uint32_t W, uint32_t H, double sigma, int which, void *s
         IMR_CHECK(launch_blur(ba, s));
         IMR_HIP_CHECK(hipMemcpyAsync(d, tmp, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     }
-    // the host weight array must outlive the copy
-    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    // staged host weights must outlive their copy
+    if (stage) IMR_HIP_CHECK(hipStreamSynchronize(s));
     return IMRESH_OK;
 }
 

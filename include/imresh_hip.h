@@ -264,7 +264,11 @@ int imresh_pack_mask(const uint8_t *mask_dev, uint32_t n, uint32_t *bits_dev, vo
 int imresh_gaussian_kernel(double sigma, float *weights_host, uint32_t capacity);
 /* cudaGaussianBlur / gaussianBlur (cudaGaussian.cu:1044-1057, gaussian.cpp:737-749),
  * the horizontal pass (cudaGaussian.cu:655-726) and the vertical pass
- * (cudaGaussian.cu:993-1041); in place, clamp-to-edge, up to 63 taps. */
+ * (cudaGaussian.cu:993-1041); in place, clamp-to-edge, up to 63 taps.
+ * Each sigma's weights go into a per-device table of 128 slots on its first
+ * use (as GaussianKernelGpuBuffer, cudaGaussian.cu:64-298); that first call
+ * synchronizes the stream, later calls with the sigma are asynchronous.
+ * With all slots taken, calls with a new sigma synchronize (no abort). */
 int imresh_gaussian_blur(float *data_dev, uint32_t width, uint32_t height, double sigma,
                          void *stream);
 int imresh_gaussian_blur_horizontal(float *data_dev, uint32_t width, uint32_t height,

@@ -232,6 +232,31 @@ def test_blur_bit_exact_vs_oracle(h, w):
             assert np.array_equal(got, exp), (h, w, s, which, float(np.abs(got - exp).max()))
 
 
+def test_blur_weight_table_and_overflow():
+    """The per-operator blur keeps each sigma's weights in a per-device table
+    of 128 slots (cudaGaussian.cu:64-298 keeps 128 per device); past that it
+    stages the weights per call.  Both paths, a repeated sigma, and a call on
+    a created stream are bit-exact with the oracle."""
+    rng = np.random.default_rng(128)
+    h, w = 37, 31
+    d = (rng.random((h, w), dtype=np.float32) - 0.5).astype(np.float32)
+    dd = B(d.nbytes)
+    sigmas = [float(x) for x in np.linspace(0.2, 9.5, 140)] + [3.0, 3.0, 0.2]
+    st = im.stream_create()
+    try:
+        for i, s in enumerate(sigmas):
+            which, fn = (("h", "imresh_gaussian_blur_horizontal"), ("v", "imresh_gaussian_blur_vertical"),
+                         ("hv", "imresh_gaussian_blur"))[i % 3]
+            dd.upload(d)
+            stream = st if i % 2 else None
+            im.check(getattr(L(), fn)(dd.ptr, w, h, s, stream), fn)
+            im.sync(stream)
+            got = dd.download(np.float32, (h, w))
+            assert np.array_equal(got, oracle.blur(d, s, which)), (i, s, which)
+    finally:
+        im.stream_destroy(st)
+
+
 def test_blur_dirac_deltas_kat():
     """testGaussian.cpp:217-337: spaced deltas reproduce the weights."""
     for s in (0.1, 0.5, 1, 1.3, 1.7, 2, 3, 4):
