@@ -28,6 +28,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "engine.hpp"
@@ -656,31 +657,45 @@ int hio_error(const float *g, const void *mask, int kind, uint32_t n, int invert
 constexpr int kWeightSlots = 128, kWeightSlotFloats = 128;
 struct WeightTable {
     float *dev = nullptr;
-    std::map<uint64_t, int> slot;
+    int used = 0;
+    std::map<uint64_t, std::pair<int, bool>> slot;   // sigma bits -> {slot, uploaded}
 };
 std::mutex g_wt_mtx;
 std::map<int, WeightTable> g_wt;
 
+// *out = sigma's weights in the table, or nullptr when the caller must stage
+// them itself (table full, or another thread's upload of sigma still
+// running).  The upload and its stream synchronization run outside the lock.
 int cached_weights(int dev, double sigma, const float *host, hipStream_t s, const float **out)
 {
-    std::lock_guard<std::mutex> lk(g_wt_mtx);
-    WeightTable &t = g_wt[dev];
     uint64_t key;
     std::memcpy(&key, &sigma, sizeof key);
-    auto it = t.slot.find(key);
-    if (it != t.slot.end()) {
-        *out = t.dev + (size_t) it->second * kWeightSlotFloats;
-        return IMRESH_OK;
-    }
     *out = nullptr;
-    if ((int) t.slot.size() >= kWeightSlots) return IMRESH_OK;
-    if (t.dev == nullptr)
-        IMR_HIP_CHECK(hipMalloc(&t.dev, sizeof(float) * kWeightSlots * kWeightSlotFloats));
-    const int idx = (int) t.slot.size();
-    float *p = t.dev + (size_t) idx * kWeightSlotFloats;
-    IMR_HIP_CHECK(hipMemcpyAsync(p, host, sizeof(float) * kWeightSlotFloats, hipMemcpyHostToDevice, s));
-    IMR_HIP_CHECK(hipStreamSynchronize(s));
-    t.slot.emplace(key, idx);
+    float *p;
+    {
+        std::lock_guard<std::mutex> lk(g_wt_mtx);
+        WeightTable &t = g_wt[dev];
+        auto it = t.slot.find(key);
+        if (it != t.slot.end()) {
+            if (it->second.second) *out = t.dev + (size_t) it->second.first * kWeightSlotFloats;
+            return IMRESH_OK;
+        }
+        if (t.used >= kWeightSlots) return IMRESH_OK;
+        if (t.dev == nullptr)
+            IMR_HIP_CHECK(hipMalloc(&t.dev, sizeof(float) * kWeightSlots * kWeightSlotFloats));
+        const int idx = t.used++;
+        t.slot.emplace(key, std::make_pair(idx, false));
+        p = t.dev + (size_t) idx * kWeightSlotFloats;
+    }
+    hipError_t e = hipMemcpyAsync(p, host, sizeof(float) * kWeightSlotFloats, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lk(g_wt_mtx);
+    WeightTable &t = g_wt[dev];
+    if (e != hipSuccess) {
+        t.slot.erase(key);   // the slot stays unused; a later call retries in another
+        return from_hip(e);
+    }
+    t.slot[key].second = true;
     *out = p;
     return IMRESH_OK;
 }

@@ -232,6 +232,43 @@ def test_blur_bit_exact_vs_oracle(h, w):
             assert np.array_equal(got, exp), (h, w, s, which, float(np.abs(got - exp).max()))
 
 
+def test_blur_concurrent_threads():
+    """Four host threads, each on its own stream, blur with the same new
+    sigmas at once (first uploads race for the same table slots); every
+    result is bit-exact with the oracle."""
+    import threading
+    rng = np.random.default_rng(4)
+    h, w = 64, 96
+    d = (rng.random((h, w), dtype=np.float32) - 0.5).astype(np.float32)
+    sigmas = [0.35 + 0.0137 * i for i in range(24)]
+    exp = {s: oracle.blur(d, s, "hv") for s in sigmas}
+    errors = []
+
+    def worker(k):
+        st = im.stream_create()
+        dd = B(d.nbytes)
+        try:
+            for s in sigmas[k:] + sigmas[:k]:
+                dd.upload(d)
+                im.check(L().imresh_gaussian_blur(dd.ptr, w, h, s, st), "blur")
+                im.sync(st)
+                if not np.array_equal(dd.download(np.float32, (h, w)), exp[s]):
+                    errors.append((k, s))
+        except Exception as e:   # reported below, from the main thread
+            errors.append((k, repr(e)))
+        finally:
+            dd.close()
+            im.stream_destroy(st)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors[:4]
+
+
 def test_blur_weight_table_and_overflow():
     """The per-operator blur keeps each sigma's weights in a per-device table
     of 128 slots (cudaGaussian.cu:64-298 keeps 128 per device); past that it
