@@ -332,8 +332,10 @@ static int server_submit_impl(imresh_server *h, float *io, uint32_t width, uint3
     if (!imr::line_supported((int) width) || !imr::line_supported((int) height))
         return IMRESH_ERR_UNSUPPORTED_SIZE;
     Server &s = h->s;
+    imr::Params q{};
+    IMR_CHECK(imr::normalize(params, q));
     auto *r = new imr::Request();
-    IMR_CHECK(imr::normalize(params, r->q));
+    r->q = q;
     r->io = io;
     r->W = width;
     r->H = height;

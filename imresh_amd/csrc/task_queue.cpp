@@ -80,19 +80,31 @@ int queue_init(int per_device)
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev <= 0) return IMRESH_ERR_NO_DEVICE;
     if (per_device <= 0) per_device = 3;   // taskQueue.cu:217
+    int caller_dev = 0;
+    e = hipGetDevice(&caller_dev);
+    if (e != hipSuccess) return imr::from_hip(e);
     g_q.closing = false;
     g_q.slots.clear();
+    // streams on every device; on a failure those created so far are
+    // destroyed and the queue stays uninitialized
+    auto unwind = [&](hipError_t err) {
+        for (auto &s : g_q.slots) hipStreamDestroy(s.stream);
+        g_q.slots.clear();
+        hipSetDevice(caller_dev);
+        return imr::from_hip(err);
+    };
     for (int d = 0; d < ndev; ++d) {
-        hipSetDevice(d);
+        e = hipSetDevice(d);
+        if (e != hipSuccess) return unwind(e);
         for (int j = 0; j < per_device; ++j) {
             Slot s;
             s.device = d;
             e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
-            if (e != hipSuccess) return imr::from_hip(e);
+            if (e != hipSuccess) return unwind(e);
             g_q.slots.push_back(std::move(s));
         }
     }
-    hipSetDevice(0);
+    hipSetDevice(caller_dev);   // the caller's current device is left as it was
     g_q.capacity = g_q.slots.size();
     g_q.running = true;
     for (size_t i = 0; i < g_q.slots.size(); ++i) g_q.slots[i].worker = std::thread(worker_main, i);

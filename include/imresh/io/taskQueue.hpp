@@ -3,7 +3,9 @@
 // an MI355X stream slot and then calls writeOutFunc(ioData, W, H, fileName)
 // on a worker thread, like the reference.  Parameter defaults that are <= 0
 // (or 0 for counts) follow cudaShrinkWrap (cudaShrinkWrap.cu:193-200), which
-// the reference queue calls.
+// the reference queue calls.  addTask before taskQueueInit (the reference
+// asserts, taskQueue.cu:163) drops the task without calling writeOutFunc;
+// the C entry point imresh_add_task reports IMRESH_ERR_NOT_INITIALIZED.
 #pragma once
 
 #include <functional>
