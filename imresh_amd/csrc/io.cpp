@@ -126,6 +126,10 @@ bool png_read(const char *path, std::vector<float> &out, uint32_t &W, uint32_t &
     const size_t row_bytes = ((size_t) w * bits_px + 7) / 8;
     const size_t bpp = bits_px >= 8 ? bits_px / 8 : 1;   // filter byte distance
     if ((size_t) h > std::numeric_limits<size_t>::max() / (row_bytes + 1)) return false;
+    // deflate expands at most 1032:1, so a header that asks for more than
+    // the compressed data can hold is corrupt: rejected before the buffer is
+    // allocated (and zero-filled) rather than after inflating fails
+    if ((row_bytes + 1) * h > idat.size() * 1032 + 1024) return false;
     std::vector<uint8_t> raw((row_bytes + 1) * h);
     uLongf raw_len = (uLongf) raw.size();
     if (uncompress(raw.data(), &raw_len, idat.data(), (uLong) idat.size()) != Z_OK || raw_len != raw.size())
@@ -291,13 +295,16 @@ bool tiff_read(const char *path, std::vector<float> &out, uint32_t &W, uint32_t 
     if ((uint64_t) w * h > (1ull << 31)) return false;
     if (offsets.empty() || offsets.size() != counts.size()) return false;
     const size_t need = (size_t) w * h * 4;
-    std::vector<uint8_t> buf;
-    buf.reserve(need);
+    size_t have = 0;   // strip bytes, all inside the file
     for (size_t s = 0; s < offsets.size(); ++s) {
         if ((size_t) offsets[s] + counts[s] > f.size()) return false;
-        buf.insert(buf.end(), f.begin() + offsets[s], f.begin() + offsets[s] + counts[s]);
+        have += counts[s];
     }
-    if (buf.size() < need) return false;
+    if (have < need) return false;
+    std::vector<uint8_t> buf;
+    buf.reserve(have);
+    for (size_t s = 0; s < offsets.size(); ++s)
+        buf.insert(buf.end(), f.begin() + offsets[s], f.begin() + offsets[s] + counts[s]);
     (void) rps;
     out.resize((size_t) w * h);
     for (size_t i = 0; i < out.size(); ++i) {

@@ -298,6 +298,42 @@ def test_tiff_float_reader(tmp_path, big_endian, rps):
     assert np.array_equal(im.read_image(p).view(np.uint32), a.view(np.uint32))
 
 
+def _peak_rss_mib():
+    import resource
+    return resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1024
+
+
+def test_corrupt_sizes_rejected_before_allocating(tmp_path):
+    """A header that claims 40000 x 40000 pixels over a few bytes of data is
+    rejected at once: the PNG reader bounds the inflated size by deflate's
+    1032:1 ratio before allocating it, the TIFF reader checks the strips
+    hold W x H floats before reserving them (a 12.8 GB zero-filled buffer
+    otherwise)."""
+    import time
+    p = str(tmp_path / "big.png")
+
+    def chunk(t, body):
+        return struct.pack(">I", len(body)) + t + body + struct.pack(">I", zlib.crc32(t + body))
+
+    ihdr = struct.pack(">IIBBBBB", 40000, 40000, 16, 6, 0, 0, 0)
+    data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(b"\0" * 64)) + chunk(b"IEND", b"")
+    open(p, "wb").write(data)
+    rss0, t0 = _peak_rss_mib(), time.perf_counter()
+    assert read(p)[0] == 10
+    t = str(tmp_path / "big.tif")
+    write_tiff(t, np.ones((2, 2), np.float32), rows_per_strip=2)
+    d = bytearray(open(t, "rb").read())
+    ifd = struct.unpack("<I", d[4:8])[0]
+    for i in range(struct.unpack("<H", d[ifd:ifd + 2])[0]):
+        e = ifd + 2 + 12 * i
+        if struct.unpack("<H", d[e:e + 2])[0] in (256, 257):   # width, height
+            d[e + 8:e + 12] = struct.pack("<I", 40000)
+    open(t, "wb").write(bytes(d))
+    assert read(t, "tiff")[0] == 10
+    assert time.perf_counter() - t0 < 2.0
+    assert _peak_rss_mib() - rss0 < 512
+
+
 def test_tiff_unsupported_layouts(tmp_path):
     p = str(tmp_path / "t.tif")
     write_tiff(p, np.ones((4, 4), np.float32), bits=16, fmt=1)
