@@ -93,12 +93,19 @@ def cpu_baseline(M: np.ndarray, M256: np.ndarray, M8=None, budget_s: float = 12.
     out = {"value": round(done / dt, 4), "unit": "HIO iters/s", "cores": oracle.num_threads(),
            "kind": "port",
            "sample": f"{done} HIO iterations (constraint, FFT, modulus, IFFT) of the 2048x2048 "
-                     f"workload on the CPU oracle; FFT single-threaded as the reference, "
+                     f"workload on the CPU oracle; FFT single-threaded as the reference's FFTW "
+                     f"plans but the project's own CPU FFT (FFTW3f is not installed), "
                      f"elementwise loops on {oracle.num_threads()} OpenMP threads; "
                      f"{dt:.1f} s",
            "recon_256_s": round(dt256, 3),
            "recon_256_note": "one full 20x20 reconstruction of the 256x256 workload on the CPU oracle",
-           "per_op_ms": cpu_per_op(g, gp, mask, M)}
+           "per_op_ms": cpu_per_op(g, gp, mask, M),
+           "per_op_ms_reference": cpu_ref_per_op(g, mask, M),
+           "fft2d_numpy_ms": _time_ms(lambda: np.fft.fft2(g)),
+           "per_op_note": "per_op_ms: the oracle's restatement; per_op_ms_reference: the reference's "
+                          "own CPU sources compiled unchanged (no FFT: FFTW is absent); "
+                          "fft2d_numpy_ms: numpy's pocketfft (float64, one thread) on the same "
+                          "plane, for scale against the oracle's FFT"}
     if M8 is not None:
         out["large_8192"] = cpu_large(M8)
     return out
@@ -113,21 +120,69 @@ def _time_ms(fn) -> float:
 def cpu_per_op(g, gp, mask, M) -> dict:
     """Each operator of one HIO iteration and of one cycle's mask update,
     timed once on the CPU oracle at the workload's size (the reference's CPU
-    operators, shrinkWrap.cpp:235-301), for comparison with the GPU's fused
-    kernels: the FFTs single-threaded, the rest on the OpenMP threads."""
+    operators, shrinkWrap.cpp:235-301), called directly on preallocated
+    buffers so only the operator is timed: the FFTs single-threaded, the
+    rest on the OpenMP threads."""
+    import ctypes
+
     import oracle
-    absg = oracle.complex_norm(g)
-    thr = 0.2 * oracle.vmax(absg)
+    L = oracle.lib()
+    h, w = mask.shape
+    n = w * h
+    gf = np.ascontiguousarray(g, dtype=np.complex64).view(np.float32)
+    gpf = np.ascontiguousarray(gp, dtype=np.complex64).view(np.float32).copy()
+    mk = np.ascontiguousarray(mask, dtype=np.float32)
+    Mf = np.ascontiguousarray(M, dtype=np.float32)
+    out_c = np.empty_like(gf)
+    absg = np.empty(n, np.float32)
+    L.orc_complex_norm_f32(absg.ctypes.data, gf.ctypes.data, n)
+    thr = np.float32(0.2) * np.float32(L.orc_vmax_f32(absg.ctypes.data, n))
+    blur_buf, cut_buf, mod_buf = absg.copy(), absg.copy(), gf.copy()
+    tot, nm = ctypes.c_float(), ctypes.c_float()
     return {
-        "fft2d_forward": _time_ms(lambda: oracle.fft2d(gp, -1)),
-        "fft2d_inverse": _time_ms(lambda: oracle.fft2d(g, +1)),
-        "hio_constraint": _time_ms(lambda: oracle.hio_constraint(gp, g, mask, 0.9)),
-        "apply_modulus": _time_ms(lambda: oracle.apply_modulus(g, M)),
-        "complex_norm": _time_ms(lambda: oracle.complex_norm(g)),
-        "gaussian_blur_sigma3": _time_ms(lambda: oracle.blur(absg, 3.0)),
-        "vector_max": _time_ms(lambda: oracle.vmax(absg)),
-        "cutoff": _time_ms(lambda: oracle.cutoff(absg, thr)),
-        "hio_error": _time_ms(lambda: oracle.hio_error(g, mask)),
+        "fft2d_forward": _time_ms(lambda: L.orc_fft2d_f32(out_c.ctypes.data, gpf.ctypes.data, w, h, -1)),
+        "fft2d_inverse": _time_ms(lambda: L.orc_fft2d_f32(out_c.ctypes.data, gf.ctypes.data, w, h, +1)),
+        "hio_constraint": _time_ms(lambda: L.orc_hio_constraint_f32(gpf.ctypes.data, gf.ctypes.data,
+                                                                    mk.ctypes.data, n, 0.9)),
+        "apply_modulus": _time_ms(lambda: L.orc_apply_modulus_f32(mod_buf.ctypes.data, Mf.ctypes.data, n)),
+        "complex_norm": _time_ms(lambda: L.orc_complex_norm_f32(absg.ctypes.data, gf.ctypes.data, n)),
+        "gaussian_blur_sigma3": _time_ms(lambda: L.orc_blur_f32(blur_buf.ctypes.data, w, h, 3.0)),
+        "vector_max": _time_ms(lambda: L.orc_vmax_f32(absg.ctypes.data, n)),
+        "cutoff": _time_ms(lambda: L.orc_cutoff_f32(cut_buf.ctypes.data, n, thr, 1.0, 0.0)),
+        "hio_error": _time_ms(lambda: L.orc_hio_error_f32(gf.ctypes.data, mk.ctypes.data, n, 0,
+                                                          ctypes.byref(tot), ctypes.byref(nm))),
+    }
+
+
+def cpu_ref_per_op(g, mask, M):
+    """The same operators from the reference's own CPU sources
+    (gaussian.cpp, vectorReduce.cpp, vectorElementwise.cpp,
+    hybridInputOutput.cpp compiled unchanged into oracle/_ref), timed once;
+    None where that library was not built (it needs the reference sources)."""
+    import ctypes
+
+    import oracle
+    r = oracle.ref()
+    if r is None:
+        return None
+    g = np.ascontiguousarray(g, dtype=np.complex64)
+    gf = g.view(np.float32)
+    mk = np.ascontiguousarray(mask, dtype=np.float32)
+    Mf = np.ascontiguousarray(M, dtype=np.float32)
+    n = mk.size
+    h, w = mk.shape
+    absg = np.empty(n, np.float32)
+    r.ref_complex_norm_f32(absg.ctypes.data, gf.ctypes.data, n)
+    blur_buf = absg.copy()
+    mod_buf = gf.copy()
+    tot, nm = ctypes.c_float(), ctypes.c_float()
+    return {
+        "complex_norm": _time_ms(lambda: r.ref_complex_norm_f32(absg.ctypes.data, gf.ctypes.data, n)),
+        "apply_modulus": _time_ms(lambda: r.ref_apply_modulus_f32(mod_buf.ctypes.data, Mf.ctypes.data, n)),
+        "gaussian_blur_sigma3": _time_ms(lambda: r.ref_blur_f32(blur_buf.ctypes.data, w, h, 3.0)),
+        "vector_max": _time_ms(lambda: r.ref_vmax_f32(absg.ctypes.data, n)),
+        "hio_error": _time_ms(lambda: r.ref_hio_error_f32(gf.ctypes.data, mk.ctypes.data, n, 0,
+                                                          ctypes.byref(tot), ctypes.byref(nm))),
     }
 
 
