@@ -372,20 +372,30 @@ int FN(orc_fft2d)(T *outp, const T *inp, unsigned w, unsigned h, int sign)
     FN(plan_init)(&pw, w, sign);
     FN(plan_init)(&ph, h, sign);
     const unsigned nmax = w > h ? w : h;
+    /* columns are copied out and back ORC_COL_BLOCK at a time, so every
+     * cache line of a row serves several columns; each line's arithmetic is
+     * the same as one column at a time */
+    enum { ORC_COL_BLOCK = 16 };
     #pragma omp parallel num_threads(g_fft_threads)
     {
         FN(cplx) *tmp = (FN(cplx) *) malloc(sizeof(FN(cplx)) * nmax);
-        FN(cplx) *col = (FN(cplx) *) malloc(sizeof(FN(cplx)) * nmax);
+        FN(cplx) *col = (FN(cplx) *) malloc(sizeof(FN(cplx)) * (size_t) h * ORC_COL_BLOCK);
         #pragma omp for
         for (int y = 0; y < (int) h; ++y) {
             FN(fft_line)(in + (size_t) y * w, 1, tmp, w, sign, &pw);
             memcpy(out + (size_t) y * w, tmp, sizeof(FN(cplx)) * w);
         }
         #pragma omp for
-        for (int x = 0; x < (int) w; ++x) {
-            for (unsigned y = 0; y < h; ++y) col[y] = out[(size_t) y * w + x];
-            FN(fft_line)(col, 1, tmp, h, sign, &ph);
-            for (unsigned y = 0; y < h; ++y) out[(size_t) y * w + x] = tmp[y];
+        for (int x0 = 0; x0 < (int) w; x0 += ORC_COL_BLOCK) {
+            const unsigned nb = w - (unsigned) x0 < ORC_COL_BLOCK ? w - (unsigned) x0 : ORC_COL_BLOCK;
+            for (unsigned y = 0; y < h; ++y)
+                for (unsigned b = 0; b < nb; ++b) col[(size_t) b * h + y] = out[(size_t) y * w + x0 + b];
+            for (unsigned b = 0; b < nb; ++b) {
+                FN(fft_line)(col + (size_t) b * h, 1, tmp, h, sign, &ph);
+                memcpy(col + (size_t) b * h, tmp, sizeof(FN(cplx)) * h);
+            }
+            for (unsigned y = 0; y < h; ++y)
+                for (unsigned b = 0; b < nb; ++b) out[(size_t) y * w + x0 + b] = col[(size_t) b * h + y];
         }
         free(tmp); free(col);
     }
