@@ -366,6 +366,27 @@ def test_fft2d_vs_numpy(h, w):
     assert np.linalg.norm(inv - exp) / np.linalg.norm(exp) < tol
 
 
+@pytest.mark.parametrize("h,w", [(64, 64), (96, 200), (1000, 12), (2048, 2048), (8192, 8)])
+def test_fft2d_and_modulus_out_of_place(h, w):
+    """out != in gives the in-place result bit for bit and leaves the input
+    untouched (imresh_fft2d and imresh_apply_complex_modulus accept both)."""
+    rng = np.random.default_rng(h + w)
+    x = (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)
+    M = rng.random((h, w), dtype=np.float32)
+    for direction in (-1, +1):
+        src, dst, ip = B.from_array(x), B(x.nbytes), B.from_array(x)
+        im.check(L().imresh_fft2d(dst.ptr, src.ptr, w, h, direction, None))
+        im.check(L().imresh_fft2d(ip.ptr, ip.ptr, w, h, direction, None))
+        assert np.array_equal(dst.download(np.complex64, (h, w)).view(np.uint32),
+                              ip.download(np.complex64, (h, w)).view(np.uint32))
+        assert np.array_equal(src.download(np.complex64, (h, w)).view(np.uint32), x.view(np.uint32))
+    src, dst, dM = B.from_array(x), B(x.nbytes), B.from_array(M)
+    im.check(L().imresh_apply_complex_modulus(dst.ptr, src.ptr, dM.ptr, h * w, None))
+    exp = oracle.apply_modulus(x.reshape(-1), M.reshape(-1))
+    assert np.array_equal(dst.download(np.complex64, h * w).view(np.uint32), exp.view(np.uint32))
+    assert np.array_equal(src.download(np.complex64, (h, w)).view(np.uint32), x.view(np.uint32))
+
+
 def test_fft2d_closed_forms():
     h, w = 64, 128
     d = np.zeros((h, w), np.complex64)
