@@ -328,9 +328,10 @@ int time_variant(bool row, int var, Plan2D p, Workspace *w, hipStream_t s, float
     IMR_HIP_CHECK(hipEventCreate(&e0));
     IMR_HIP_CHECK(hipEventCreate(&e1));
     int st = IMRESH_OK;
-    // best of two trials of 8 back-to-back launches on a refilled state
-    // (fill values are below 1; 8 unnormalized row launches stay finite)
-    constexpr int reps = 8, trials = 2;
+    // one trial of 8 back-to-back launches on a refilled state (fill values
+    // are below 1; 8 unnormalized row launches stay finite); the caller
+    // repeats the trials in interleaved rounds
+    constexpr int reps = 8, trials = 1;
     float best = 1e30f;
     for (int k = 0; k < trials && st == IMRESH_OK; ++k) {
         st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, w->W, s);
@@ -366,18 +367,25 @@ int autotune(int device, int W, int H, hipStream_t s, Workspace *w)
         rv = std::min(std::max(rv, 0), std::max(nr - 1, 0));
         cv = std::min(std::max(cv, 0), std::max(nc - 1, 0));
     } else if (!(off != nullptr && off[0] == '0') && (nr > 1 || nc > 1)) {
-        float best = 1e30f;
-        for (int v = 0; v < nr && nr > 1; ++v) {
-            float ms;
-            IMR_CHECK(time_variant(true, v, p, w, s, &ms));
-            if (ms < best) { best = ms; rv = v; }
+        // every variant once per round, best of kTuneRounds rounds: timing
+        // the variants one after the other would bias against the first
+        // ones, which can run while the GPU is still raising its clocks
+        constexpr int kTuneRounds = 3;
+        std::vector<float> br(nr > 1 ? nr : 0, 1e30f), bc(nc > 1 ? nc : 0, 1e30f);
+        for (int round = 0; round < kTuneRounds; ++round) {
+            for (int v = 0; v < (int) br.size(); ++v) {
+                float ms;
+                IMR_CHECK(time_variant(true, v, p, w, s, &ms));
+                br[v] = std::min(br[v], ms);
+            }
+            for (int v = 0; v < (int) bc.size(); ++v) {
+                float ms;
+                IMR_CHECK(time_variant(false, v, p, w, s, &ms));
+                bc[v] = std::min(bc[v], ms);
+            }
         }
-        best = 1e30f;
-        for (int v = 0; v < nc && nc > 1; ++v) {
-            float ms;
-            IMR_CHECK(time_variant(false, v, p, w, s, &ms));
-            if (ms < best) { best = ms; cv = v; }
-        }
+        if (!br.empty()) rv = (int) (std::min_element(br.begin(), br.end()) - br.begin());
+        if (!bc.empty()) cv = (int) (std::min_element(bc.begin(), bc.end()) - bc.begin());
     }
     set_tuned_variants(device, W, H, rv, cv);
     return IMRESH_OK;
