@@ -102,6 +102,7 @@ void ws_release(Workspace *w)
     if (w->base) hipFree(w->base);
     if (w->weights) hipFree(w->weights);
     if (w->host_stage) hipHostFree(w->host_stage);
+    if (w->host_wpin) hipHostFree(w->host_wpin);
     if (w->ready) hipEventDestroy(w->ready);
     if (w->gexec) hipGraphExecDestroy(w->gexec);
     delete w;
@@ -274,8 +275,11 @@ int padded_weights(const Params &q, std::vector<float> &rows, int *R)
     return IMRESH_OK;
 }
 
-// the workspace's weights table for q (uploaded once per parameter set; the
-// host copy lives in the workspace, so the upload needs no synchronization)
+// the workspace's weights table for q, uploaded once per parameter set from
+// pinned memory kept with the workspace, so the copy is asynchronous (a copy
+// from pageable memory would hold the calling thread until the stream's
+// earlier work had finished).  The staging is rewritten only once the
+// workspace's previous user has finished, whose copy may still read it.
 int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
 {
     if (w->weights && w->w_sigma0 == q.sigma0 && w->w_dsigma == q.dsigma && w->w_ncycles == q.n_cycles)
@@ -283,6 +287,15 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
     std::vector<float> &host = w->host_weights;
     int R = 0;
     IMR_CHECK(padded_weights(q, host, &R));
+    if (w->pending && hipEventQuery(w->ready) != hipSuccess) IMR_HIP_CHECK(hipEventSynchronize(w->ready));
+    if (w->host_wpin_n < host.size()) {
+        if (w->host_wpin) hipHostFree(w->host_wpin);
+        w->host_wpin = nullptr;
+        w->host_wpin_n = 0;
+        IMR_HIP_CHECK(hipHostMalloc((void **) &w->host_wpin, sizeof(float) * host.size(), hipHostMallocDefault));
+        w->host_wpin_n = host.size();
+    }
+    std::memcpy(w->host_wpin, host.data(), sizeof(float) * host.size());
     const unsigned rows = std::max(1u, q.n_cycles);
     if (w->wcap < rows) {
         if (w->weights) {
@@ -293,7 +306,7 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
         w->wcap = rows;
         IMR_HIP_CHECK(hipMalloc(&w->weights, sizeof(float) * kBlurWeightRow * w->wcap));
     }
-    IMR_HIP_CHECK(hipMemcpyAsync(w->weights, host.data(), sizeof(float) * host.size(),
+    IMR_HIP_CHECK(hipMemcpyAsync(w->weights, w->host_wpin, sizeof(float) * host.size(),
                                  hipMemcpyHostToDevice, s));
     w->wR = R;
     w->w_sigma0 = q.sigma0;

@@ -42,6 +42,8 @@ struct Workspace {
     float w_sigma0 = -1.f, w_dsigma = -1.f;
     unsigned w_ncycles = 0;
     std::vector<float> host_weights;   // host copy of the weights table
+    float *host_wpin = nullptr;        // pinned staging of that table (the upload's source)
+    size_t host_wpin_n = 0;            // its capacity in floats
     float *host_stage = nullptr;  // pinned, 64 floats + readback
     hipEvent_t ready = nullptr;   // recorded when the last user's work is queued
     bool pending = false;

@@ -18,6 +18,12 @@
 //     in the caller's buffer, the result record, the latency (submit to
 //     completion), the callback.
 //
+// The scheduler thread never waits on a lane: host buffers go through
+// pinned staging (a copy from or to pageable memory would hold the thread
+// until the lane's earlier work, other requests' cycles included, had
+// finished), and a new size's kernel variants are timed on a stream of
+// their own.
+//
 // Results are those of imresh_shrinkwrap on the same input: the same
 // kernels run on the request's own state, only interleaved with others.
 #include <chrono>
@@ -58,6 +64,44 @@ struct Request {
     bool finishing = false;
     hipEvent_t fin = nullptr;
     int status = IMRESH_OK;
+    void *pin = nullptr;   // pinned staging of the host buffer (host I/O only)
+};
+
+// Pinned host buffers by power-of-two capacity, reused across requests
+// (scheduler thread only).
+struct PinnedPool {
+    std::map<size_t, std::vector<void *>> free;
+    std::vector<void *> all;
+
+    static size_t cap(size_t bytes)
+    {
+        size_t c = 64 << 10;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    void *get(size_t bytes)
+    {
+        auto &v = free[cap(bytes)];
+        if (!v.empty()) {
+            void *p = v.back();
+            v.pop_back();
+            return p;
+        }
+        void *p = nullptr;
+        if (hipHostMalloc(&p, cap(bytes), hipHostMallocDefault) != hipSuccess) return nullptr;
+        all.push_back(p);
+        return p;
+    }
+    void put(void *p, size_t bytes)
+    {
+        if (p) free[cap(bytes)].push_back(p);
+    }
+    void release()
+    {
+        for (void *p : all) hipHostFree(p);
+        all.clear();
+        free.clear();
+    }
 };
 
 }  // namespace
@@ -67,7 +111,9 @@ struct Server {
     uint64_t budget = 0;
     uint32_t max_active = 0;
     std::vector<hipStream_t> lanes;
+    hipStream_t tune = nullptr;   // kernel-variant timing of new sizes
     unsigned next_lane = 0;
+    PinnedPool pinned;
 
     std::mutex mtx;
     std::condition_variable cv;       // work arrived / stop
@@ -134,6 +180,8 @@ struct Server {
         c.result = res;
         c.latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
         if (r->w) idle[{(int) r->W, (int) r->H}].push_back(r->w);
+        pinned.put(r->pin, sizeof(float) * (size_t) r->W * r->H);
+        r->pin = nullptr;
         for (auto &e : r->inflight) put_event(e.first);
         put_event(r->fin);
         if (r->cb) r->cb(r->user, &c, r->io);
@@ -174,9 +222,16 @@ struct Server {
             r->lane = lanes[next_lane++ % lanes.size()];
             const size_t bytes = sizeof(float) * w->n;
             const float *M = r->device_io ? r->io : w->M;
-            int s2 = ws_tune(w, r->lane);
-            if (s2 == IMRESH_OK && !r->device_io)
-                s2 = from_hip(hipMemcpyAsync(w->M, r->io, bytes, hipMemcpyHostToDevice, r->lane));
+            int s2 = ws_tune(w, tune);
+            if (s2 == IMRESH_OK && !r->device_io) {
+                r->pin = pinned.get(bytes);
+                if (r->pin == nullptr) {
+                    s2 = IMRESH_ERR_OUT_OF_MEMORY;
+                } else {
+                    std::memcpy(r->pin, r->io, bytes);
+                    s2 = from_hip(hipMemcpyAsync(w->M, r->pin, bytes, hipMemcpyHostToDevice, r->lane));
+                }
+            }
             if (s2 == IMRESH_OK) s2 = recon_begin(w, M, r->q, r->lane);
             r->status = s2;
             std::lock_guard<std::mutex> lk(mtx);
@@ -211,7 +266,7 @@ struct Server {
                 float *out = r->device_io ? r->io : r->w->M;
                 int st = recon_end(r->w, out, r->lane);
                 if (st == IMRESH_OK && !r->device_io)
-                    st = from_hip(hipMemcpyAsync(r->io, r->w->M, sizeof(float) * r->w->n,
+                    st = from_hip(hipMemcpyAsync(r->pin, r->w->M, sizeof(float) * r->w->n,
                                                  hipMemcpyDeviceToHost, r->lane));
                 if (st == IMRESH_OK) st = recon_stage_results(r->w, r->lane);
                 r->fin = get_event();
@@ -248,7 +303,10 @@ struct Server {
             }
             if (gone) {
                 imresh_result res{};
-                if (r->status == IMRESH_OK) recon_read_staged(r->w, &res, nullptr);
+                if (r->status == IMRESH_OK) {
+                    recon_read_staged(r->w, &res, nullptr);
+                    if (!r->device_io) std::memcpy(r->io, r->pin, sizeof(float) * r->w->n);
+                }
                 {
                     std::lock_guard<std::mutex> lk(mtx);
                     active.erase(active.begin() + (long) i);
@@ -312,10 +370,16 @@ static int server_create_impl(uint64_t hbm_budget_bytes, uint32_t max_active, ui
         }
         s.lanes.push_back(st);
     }
+    if (hipStreamCreateWithFlags(&s.tune, hipStreamNonBlocking) != hipSuccess) {
+        for (hipStream_t x : s.lanes) hipStreamDestroy(x);
+        delete h;
+        return IMRESH_ERR_HIP;
+    }
     try {
         s.thr = std::thread([&s] { s.loop(); });
     } catch (...) {
         for (hipStream_t x : s.lanes) hipStreamDestroy(x);
+        hipStreamDestroy(s.tune);
         delete h;
         return IMRESH_ERR_INTERNAL;
     }
@@ -428,6 +492,8 @@ int imresh_server_destroy(imresh_server *h)
         for (imr::Workspace *w : kv.second) imr::ws_release(w);
     for (hipEvent_t e : s.event_pool) hipEventDestroy(e);
     for (hipStream_t st : s.lanes) hipStreamDestroy(st);
+    hipStreamDestroy(s.tune);
+    s.pinned.release();
     delete h;
     return IMRESH_OK;
 }
