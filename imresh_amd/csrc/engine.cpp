@@ -565,10 +565,16 @@ int recon_end(Workspace *w, float *io_dev, hipStream_t s)
     return launch_copy_from_real(io_dev, w->X, w->n, s);
 }
 
+// the scalars and the flags lie kStageFlags floats apart in the workspace
+// allocation (each region 256-byte aligned), so one copy stages both
+constexpr size_t kStageFlags = 64;
+
 int recon_stage_results(Workspace *w, hipStream_t s)
 {
-    IMR_HIP_CHECK(hipMemcpyAsync(w->host_stage, w->scal, 16, hipMemcpyDeviceToHost, s));
-    IMR_HIP_CHECK(hipMemcpyAsync(w->host_stage + 8, w->flags, 8, hipMemcpyDeviceToHost, s));
+    static_assert(sizeof(float) * (kStageFlags + 2) <= 4096, "staging buffer");
+    if ((const void *) w->flags != (const void *) (w->scal + kStageFlags)) return IMRESH_ERR_INTERNAL;
+    IMR_HIP_CHECK(hipMemcpyAsync(w->host_stage, w->scal, sizeof(float) * (kStageFlags + 2),
+                                 hipMemcpyDeviceToHost, s));
     return IMRESH_OK;
 }
 
@@ -576,7 +582,7 @@ void recon_read_staged(const Workspace *w, imresh_result *res, int *done)
 {
     const float *st = w->host_stage;
     int flags[2];
-    std::memcpy(flags, st + 8, 8);
+    std::memcpy(flags, st + kStageFlags, 8);
     if (res != nullptr) {
         res->cycles_run = (uint32_t) flags[1];
         res->last_error = st[1];
