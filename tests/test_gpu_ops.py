@@ -593,3 +593,62 @@ def test_pool_cap_and_release():
         assert L().imresh_set_option(2, -1) == 2
     finally:
         im.check(L().imresh_set_option(2, 32768))
+
+
+# ------------------------------------------------------------ against the reference's own code
+@pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref (the reference's CPU sources) not built")
+def test_ops_bit_exact_vs_reference_sources():
+    """The GPU operators against the reference's own CPU sources compiled
+    unchanged (gaussian.cpp, calcGaussianKernel.cpp, vectorReduce.cpp and
+    the templates of vectorElementwise.cpp / hybridInputOutput.cpp, in
+    oracle/_ref): blur (both passes and 2-D), weights, max, min, complex norm
+    and modulus projection bit for bit; sum and HIO error within 1e-6 of
+    float64 (the reference's OpenMP reductions are order-dependent)."""
+    r = oracle.ref()
+    rng = np.random.default_rng(20261202)
+    for h, w in ((37, 31), (234, 512), (100, 1021)):
+        d = (rng.random((h, w), dtype=np.float32) - 0.25).astype(np.float32)
+        dd = B(d.nbytes)
+        for s in (0.5, 1.7, 3.0, 6.0, 10.0):
+            k = im.gaussian_kernel(s)
+            kr = np.zeros(64, np.float32)
+            assert r.ref_gauss_kernel_f32(s, kr.ctypes.data, 64) == len(k)
+            assert np.array_equal(k, kr[:len(k)])
+            for fn, rf in (("imresh_gaussian_blur_horizontal", r.ref_blur_h_f32),
+                           ("imresh_gaussian_blur_vertical", r.ref_blur_v_f32),
+                           ("imresh_gaussian_blur", r.ref_blur_f32)):
+                dd.upload(d)
+                im.check(getattr(L(), fn)(dd.ptr, w, h, s, None), fn)
+                exp = d.copy()
+                rf(exp.ctypes.data, w, h, s)
+                assert np.array_equal(dd.download(np.float32, (h, w)), exp), (h, w, s, fn)
+    n = 300007
+    x = (rng.standard_normal(n) * 50).astype(np.float32)
+    assert _reduce("imresh_vector_max", x) == np.float32(r.ref_vmax_f32(x.ctypes.data, n))
+    assert _reduce("imresh_vector_min", x) == np.float32(r.ref_vmin_f32(x.ctypes.data, n))
+    y = rng.random(n, dtype=np.float32)
+    exact = float(np.sum(y.astype(np.float64)))
+    assert abs(float(_reduce("imresh_vector_sum", y)) - exact) <= 1e-6 * exact
+    assert abs(float(r.ref_vsum_f32(y.ctypes.data, n)) - exact) <= 1e-6 * exact
+    c = (rng.standard_normal(n) * 3 + 1j * rng.standard_normal(n)).astype(np.complex64)
+    c[::97] = 0
+    got = run_norm(c)
+    exp = np.empty(n, np.float32)
+    r.ref_complex_norm_f32(exp.ctypes.data, c.ctypes.data, n)
+    assert np.array_equal(got, exp)
+    M = rng.random(n, dtype=np.float32) * 4
+    dc, dM = B.from_array(c), B.from_array(M)
+    im.check(L().imresh_apply_complex_modulus(dc.ptr, dc.ptr, dM.ptr, n, None))
+    ci = c.copy()
+    r.ref_apply_modulus_f32(ci.ctypes.data, M.ctypes.data, n)
+    assert np.array_equal(dc.download(np.complex64, n).view(np.uint32), ci.view(np.uint32))
+    mask = (rng.random(n) < 0.3).astype(np.float32)
+    dg, dm = B.from_array(c), B.from_array(mask)
+    err, tot, nm = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    im.check(L().imresh_calculate_hio_error(dg.ptr, dm.ptr, n, 0, _f(err), _f(tot), _f(nm), None))
+    rt, rn = ctypes.c_float(), ctypes.c_float()
+    r.ref_hio_error_f32(c.ctypes.data, mask.ctypes.data, n, 0, ctypes.byref(rt), ctypes.byref(rn))
+    exact_tot = float(np.sum(mask.astype(np.float64) * np.abs(c.astype(np.complex128))))
+    assert nm.value == rn.value == float(mask.sum())
+    for t in (tot.value, rt.value):
+        assert abs(t - exact_tot) <= 1e-6 * exact_tot
