@@ -172,8 +172,8 @@ typedef void (*imresh_server_callback)(void *user, const imresh_server_completio
 
 /* Continuous batching on the current device: requests of any supported
  * size are admitted in arrival order while fewer than max_active run and
- * their device state fits hbm_budget_bytes (0: 200 GiB; max_active 0: 8;
- * lanes = HIP streams, 0: 8); every admitted request advances cycle by cycle
+ * their device state fits hbm_budget_bytes (0: 200 GiB; max_active 0: 16;
+ * lanes = HIP streams, 0: 16); every admitted request advances cycle by cycle
  * with its state resident on the device, interleaved with the others.  Each
  * result equals imresh_shrinkwrap on the same input. */
 int imresh_server_create(uint64_t hbm_budget_bytes, uint32_t max_active, uint32_t lanes,
