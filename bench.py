@@ -51,6 +51,9 @@ FLT_MIN = float(np.finfo(np.float32).tiny)
 #   column pass (forward cols, modulus, inverse cols) : read X 8 + M 4, write X 8
 KERNEL_BYTES = {"row_mid": 33, "col_proj": 20, "blur": 8, "row_last": 21}
 KERNEL_NAMES = ["row_mid", "col_proj", "blur", "row_last"]
+# cycle-graph modes the batched and serving legs time (1 = per-cycle HIP graph
+# replay, 0 = eager launches); --cycle-graphs narrows it to one
+GRAPH_MODES = (1, 0)
 
 
 def log(msg: str) -> None:
@@ -269,7 +272,7 @@ def batch_bench(L, sizes, mods, reps: int = 2, plain_streams: int = 3) -> dict:
     # both paths timed with per-cycle graph replay and with eager launches;
     # each reports its faster mode
     modes = {}
-    for graphs in (1, 0):
+    for graphs in GRAPH_MODES:
         im.set_option(im.OPT_CYCLE_GRAPHS, graphs)
         log(f"[bench] batch vs plain loop, cycle graphs {'on' if graphs else 'off'} (warmup first)")
         restore(outs_b)
@@ -287,7 +290,7 @@ def batch_bench(L, sizes, mods, reps: int = 2, plain_streams: int = 3) -> dict:
             plain()
             tp.append(time.perf_counter() - t0)
         modes[graphs] = (min(tb), min(tp))
-    im.set_option(im.OPT_CYCLE_GRAPHS, 1)
+    im.set_option(im.OPT_CYCLE_GRAPHS, GRAPH_MODES[0])
     gb = min(modes, key=lambda g: modes[g][0])
     gp = min(modes, key=lambda g: modes[g][1])
     tb, tp = [modes[gb][0]], [modes[gp][1]]
@@ -320,7 +323,7 @@ def _percentiles(lat):
             "p99.9_ms": round(float(np.percentile(a, 99.9)), 2), "max_ms": round(float(a[-1]), 2)}
 
 
-def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, modes=(1, 0)) -> dict:
+def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, modes=None) -> dict:
     """Serving workload: `count` requests of synthetic.serve_trace (Poisson
     arrivals at `rate`/s, sizes log-uniform in [128, 2048], 5/10/20 cycles x
     20 HIO), host buffers as the reference API takes them.  The same trace is
@@ -379,12 +382,12 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, mode
         return lat, max(t_done) - t0, ios
 
     res_s, res_q = {}, {}
-    for graphs in modes:
+    for graphs in (modes or GRAPH_MODES):
         im.set_option(im.OPT_CYCLE_GRAPHS, graphs)
         log(f"[bench] serving replay at {rate:g}/s, cycle graphs {'on' if graphs else 'off'}")
         res_s[graphs] = run_server()
         res_q[graphs] = run_queue()
-    im.set_option(im.OPT_CYCLE_GRAPHS, 1)
+    im.set_option(im.OPT_CYCLE_GRAPHS, GRAPH_MODES[0])
     gs = min(res_s, key=lambda g: res_s[g][1])
     gq = min(res_q, key=lambda g: res_q[g][1])
     lat_s, span_s, ios_s, peak, bad = res_s[gs]
@@ -565,6 +568,7 @@ def forward_model_bench(L, obj: np.ndarray, reps: int = 50) -> dict:
 
 
 def main() -> int:
+    global GRAPH_MODES
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -578,6 +582,9 @@ def main() -> int:
     ap.add_argument("--no-forward-model", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--no-per-op", action="store_true")
+    ap.add_argument("--cycle-graphs", choices=("both", "on", "off"), default="both",
+                    help="cycle-graph modes of the batched and serving legs (default: time both); "
+                         "'off' also runs the other legs with eager launches")
     ap.add_argument("--cpu-large", action="store_true",
                     help="also time the 8192x8192 workload on the CPU oracle (about a minute)")
     ap.add_argument("--batch-count", type=int, default=512)
@@ -599,6 +606,8 @@ def main() -> int:
     d_src = im.DeviceBuffer.from_array(M)
     d_io = im.DeviceBuffer(M.nbytes)
     L = im.lib()
+    GRAPH_MODES = {"both": (1, 0), "on": (1,), "off": (0,)}[args.cycle_graphs]
+    im.set_option(im.OPT_CYCLE_GRAPHS, GRAPH_MODES[0])
     p = im.default_params(n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
 
     def step():
