@@ -34,35 +34,46 @@ __device__ __forceinline__ cf blue_chirp(const Blue &b, int e)
 
 // circular convolution with the chirp: v holds the chirped, zero-padded
 // line at PM's forward first-pass input positions on entry and the result
-// at PM's inverse last-pass output positions on exit
-template <class PM, int DIR, class Lds>
+// at PM's inverse last-pass output positions on exit.  HP: load the chirp's
+// spectrum before the forward transform's exchanges, so its (L2) latency
+// overlaps them (faster for the row kernels; the column kernels, with more
+// registers live, were slower with it)
+template <class PM, int DIR, bool HP, class Lds>
 __device__ __forceinline__ void blue_conv(cf (&v)[PM::E], const cf *tw, int t, const Lds &L,
                                           const Blue &b)
 {
-    compute<PM, -1, 0>(v, tw, t);
-    passes_from<PM, -1, 1>(v, tw, t, L);
     const cf *h = b.tab + b.n + (DIR < 0 ? 0 : PM::n);
-    for_outputs<PM, -1, PM::np - 1>(t, [&](int s, int e) { v[s] = cmul(v[s], h[e]); });
+    if constexpr (HP) {
+        cf hv[PM::E];
+        for_outputs<PM, -1, PM::np - 1>(t, [&](int s, int e) { hv[s] = h[e]; });
+        compute<PM, -1, 0>(v, tw, t);
+        passes_from<PM, -1, 1>(v, tw, t, L);
+        for_outputs<PM, -1, PM::np - 1>(t, [&](int s, int e) { v[s] = cmul(v[s], hv[s]); (void) e; });
+    } else {
+        compute<PM, -1, 0>(v, tw, t);
+        passes_from<PM, -1, 1>(v, tw, t, L);
+        for_outputs<PM, -1, PM::np - 1>(t, [&](int s, int e) { v[s] = cmul(v[s], h[e]); });
+    }
     compute<PM, +1, 0>(v, tw, t);
     passes_from<PM, +1, 1>(v, tw, t, L);
 }
 
 // one length-n transform of direction DIR: src(e) / dst(e, x) for e < n
-template <class PM, int DIR, class Lds, class Src, class Dst>
+template <class PM, int DIR, bool HP, class Lds, class Src, class Dst>
 __device__ __forceinline__ void blue_fft(cf (&v)[PM::E], const cf *tw, int t, const Lds &L,
                                          const Blue &b, Src &src, Dst &dst)
 {
     const int n = b.n;
     auto in = [&](int e) { return e < n ? cmul(src(e), blue_chirp<DIR>(b, e)) : mkc(0.f, 0.f); };
     gather_global<PM, -1, 0>(v, t, in);
-    blue_conv<PM, DIR>(v, tw, t, L, b);
+    blue_conv<PM, DIR, HP>(v, tw, t, L, b);
     for_outputs<PM, +1, PM::np - 1>(t, [&](int s, int e) {
         if (e < n) dst(e, cmul(v[s], blue_chirp<DIR>(b, e)));
     });
 }
 
 // transform DA, mid(e, z, pre(e)) on its outputs, transform DB (as line_pair)
-template <class PM, int DA, int DB, class Lds, class Src, class Pre, class Mid, class Dst>
+template <class PM, int DA, int DB, bool HP, class Lds, class Src, class Pre, class Mid, class Dst>
 __device__ __forceinline__ void blue_pair(cf (&v)[PM::E], const cf *tw, int t, const Lds &L,
                                           const Blue &b, Src &src, Pre &pre, Mid &mid, Dst &dst)
 {
@@ -75,7 +86,7 @@ __device__ __forceinline__ void blue_pair(cf (&v)[PM::E], const cf *tw, int t, c
     for_outputs<PM, +1, PM::np - 1>(t, [&](int s, int e) {
         if (e < n) pf[s] = pre(e);
     });
-    blue_conv<PM, DA>(v, tw, t, L, b);
+    blue_conv<PM, DA, HP>(v, tw, t, L, b);
     for_outputs<PM, +1, PM::np - 1>(t, [&](int s, int e) {
         if (e < n) {
             const cf z = mid(e, cmul(v[s], blue_chirp<DA>(b, e)), pf[s]);
@@ -84,7 +95,7 @@ __device__ __forceinline__ void blue_pair(cf (&v)[PM::E], const cf *tw, int t, c
             v[s] = mkc(0.f, 0.f);
         }
     });
-    blue_conv<PM, DB>(v, tw, t, L, b);
+    blue_conv<PM, DB, HP>(v, tw, t, L, b);
     for_outputs<PM, +1, PM::np - 1>(t, [&](int s, int e) {
         if (e < n) dst(e, cmul(v[s], blue_chirp<DB>(b, e)));
     });
@@ -96,19 +107,20 @@ namespace imr {
 
 // direct (compiled plan of the line length) or Bluestein (BL: P is the
 // convolution plan, the line length is b.n) line transforms
-template <class P, int DIR, bool BL, class Lds, class Src, class Dst>
+template <class P, int DIR, bool BL, bool HP = false, class Lds, class Src, class Dst>
 __device__ __forceinline__ void xfft(cf (&v)[P::E], const cf *tw, int t, const Lds &L, const Blue &b,
                                      Src &src, Dst &dst)
 {
-    if constexpr (BL) blue_fft<P, DIR>(v, tw, t, L, b, src, dst);
+    if constexpr (BL) blue_fft<P, DIR, HP>(v, tw, t, L, b, src, dst);
     else line_fft<P, DIR>(v, tw, t, L, src, dst);
 }
 
-template <class P, int DA, int DB, bool BL, class Lds, class Src, class Pre, class Mid, class Dst>
+template <class P, int DA, int DB, bool BL, bool HP = false, class Lds, class Src, class Pre, class Mid,
+          class Dst>
 __device__ __forceinline__ void xpair(cf (&v)[P::E], const cf *tw, int t, const Lds &L, const Blue &b,
                                       Src &src, Pre &pre, Mid &mid, Dst &dst)
 {
-    if constexpr (BL) blue_pair<P, DA, DB>(v, tw, t, L, b, src, pre, mid, dst);
+    if constexpr (BL) blue_pair<P, DA, DB, HP>(v, tw, t, L, b, src, pre, mid, dst);
     else line_pair<P, DA, DB>(v, tw, t, L, src, pre, mid, dst);
 }
 

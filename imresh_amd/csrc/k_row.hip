@@ -73,11 +73,11 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             // (diffractionIntensity.cpp:72-77): only |z| leaves the registers
             float *Bn = a.Bn + wb;
             auto sl = [=](int e, cf x) { if (valid) at32(Bn, lo + e) = cnorm(x); };
-            xfft<P, -1, BL>(v, tw, t, L, bl, g0, sl);
+            xfft<P, -1, BL, true>(v, tw, t, L, bl, g0, sl);
         } else {
             cf *out = (cf *) a.out + wb;
             auto sl = [=](int e, cf x) { if (valid) at32(out, lo + e) = x; };
-            xfft<P, K == RK_FWD ? -1 : +1, BL>(v, tw, t, L, bl, g0, sl);
+            xfft<P, K == RK_FWD ? -1 : +1, BL, true>(v, tw, t, L, bl, g0, sl);
         }
     } else if constexpr (K == RK_INVOUT) {
         __shared__ float red_f[NL * P::T];
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
                 }
             }
         };
-        xfft<P, +1, BL>(v, tw, t, L, bl, ldX, sl);
+        xfft<P, +1, BL, true>(v, tw, t, L, bl, ldX, sl);
         if constexpr ((fl & kRowError) != 0) {
             err = block_reduce(err, red_f, [](float x, float y) { return x + y; });
             cnt = block_reduce(cnt, red_u, [](unsigned x, unsigned y) { return x + y; });
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             if (valid) at32(G, lo + e) = gp;
             return gp;
         };
-        xfft<P, -1, BL>(v, tw, t, L, bl, g0, stX);
+        xfft<P, -1, BL, true>(v, tw, t, L, bl, g0, stX);
     } else if constexpr (K == RK_MID) {
         cf *G = (cf *) a.G + wb;
         const uint8_t *mask = a.mask + wb;
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(NL *P::T) k_row(RowArgs a)
             if (valid) at32(G, lo + e) = gp;
             return gp;
         };
-        xpair<P, +1, -1, BL>(v, tw, t, L, bl, ldX, pre, mid, stX);
+        xpair<P, +1, -1, BL, true>(v, tw, t, L, bl, ldX, pre, mid, stX);
     }
 }
 
