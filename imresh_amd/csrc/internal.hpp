@@ -141,12 +141,15 @@ constexpr int kFourStepMinLine = 8192;
 
 // variant index of the Bluestein kernels: kBlueVariant + {0: the plan's
 // column tile, 1: half as many columns per workgroup, 2 / 3: the same with
-// the prefetching junction}; rows use kBlueVariant (+1 prefetching).  Sizes
-// without a plan are not tuned one by one (a server meets hundreds of them):
+// the prefetching junction}; rows use kBlueVariant + {0: the plan's rows
+// per workgroup, 1: prefetching, 2: half as many rows}.  Sizes without a
+// plan are not tuned one by one (a server meets hundreds of them):
 // half-width column tiles measured fastest on every Bluestein size tried
-// (300-2039: 1-1.4x), so they are the fixed choice.
+// (300-2039: 1-1.4x), and half-height row groups spread a mid-size image
+// over more CUs (700: row pass 19.5 -> 17.6 us), so both are fixed choices.
 constexpr int kBlueVariant = 1000;
 constexpr int kBlueColDefault = 1;
+constexpr int kBlueRowDefault = 2;
 constexpr int kMaxLine = 4096;   // longest line a Bluestein plan serves
 
 struct Plan2D {

@@ -239,6 +239,8 @@ int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
     if constexpr (blue_length<N>()) {
         if (var == kBlueVariant) return launch_variant<P16, P16::L, false, true>(mode, a, s);
         if (var == kBlueVariant + 1) return launch_variant<P16, P16::L, true, true>(mode, a, s);
+        if (var == kBlueVariant + 2)
+            return launch_variant<P16, (P16::L > 1 ? P16::L / 2 : 1), false, true>(mode, a, s);
     }
     constexpr int ns = row_shape_count<N, Rs...>();
     const bool pf = var >= ns;
@@ -259,6 +261,7 @@ template <int N, int... Rs>
 int row_variant_rows(int var)
 {
     using P16 = PlanE<16, N, Rs...>;
+    if (var == kBlueVariant + 2) return P16::L > 1 ? P16::L / 2 : 1;
     if (var >= kBlueVariant) return P16::L;
     constexpr int ns = row_shape_count<N, Rs...>();
     const int shape = var >= ns ? var - ns : var;

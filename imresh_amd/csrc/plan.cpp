@@ -268,7 +268,7 @@ int row_grid(const Plan2D &p)
 {
     int L = 1;
     const int n = p.row.blue ? p.row.m : p.W;
-    const int v = p.row.blue ? kBlueVariant : p.rowVar;
+    const int v = p.row.blue ? kBlueVariant + kBlueRowDefault : p.rowVar;
     switch (p.row.group) {
     case 0: L = row_rows_g0(n, v); break;
     case 1: L = row_rows_g1(n, v); break;
@@ -288,7 +288,7 @@ int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
     a.len = p.W;
     a.bs = p.row.bs;
     const int n = p.row.blue ? p.row.m : p.W;
-    const int v = p.row.blue ? kBlueVariant : p.rowVar;
+    const int v = p.row.blue ? kBlueVariant + kBlueRowDefault : p.rowVar;
     switch (p.row.group) {
     case 0: return launch_row_g0(mode, v, n, p.row, a, s);
     case 1: return launch_row_g1(mode, v, n, p.row, a, s);
