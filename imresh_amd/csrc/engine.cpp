@@ -633,23 +633,85 @@ int check_n(const void *p, uint32_t n)
     return IMRESH_OK;
 }
 
-int read_scalar(const float *dev, float *host, hipStream_t s)
+// Scratch of the synchronous per-operator reductions, reused across calls:
+// per device a free list of slots, each a device block (the partial sums)
+// and a pinned host block (the result), so a call allocates nothing after
+// its device's first calls and reads its result back without staging
+// through pageable memory.  Concurrent calls take different slots.
+struct ReduceSlot {
+    void *dev = nullptr;
+    float *host = nullptr;
+};
+constexpr size_t kReduceSlotBytes = sizeof(float) * (2 * kReduceBlocks + 64);
+std::mutex g_rs_mtx;
+std::map<int, std::vector<ReduceSlot>> g_rs_free;
+
+int reduce_slot_get(int dev, ReduceSlot *out)
 {
-    IMR_HIP_CHECK(hipMemcpyAsync(host, dev, sizeof(float), hipMemcpyDeviceToHost, s));
-    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    {
+        std::lock_guard<std::mutex> lk(g_rs_mtx);
+        auto &v = g_rs_free[dev];
+        if (!v.empty()) {
+            *out = v.back();
+            v.pop_back();
+            return IMRESH_OK;
+        }
+    }
+    ReduceSlot r;
+    IMR_HIP_CHECK(hipMalloc(&r.dev, kReduceSlotBytes));
+    const hipError_t e = hipHostMalloc((void **) &r.host, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hipFree(r.dev);
+        return from_hip(e);
+    }
+    *out = r;
     return IMRESH_OK;
 }
+
+void reduce_slot_put(int dev, const ReduceSlot &r)
+{
+    std::lock_guard<std::mutex> lk(g_rs_mtx);
+    g_rs_free[dev].push_back(r);
+}
+
+// a slot held for one call and returned on every exit path; a call that
+// did not reach its own synchronization (an error after a launch) first
+// drains its stream, so no queued kernel still writes the slot
+struct SlotHold {
+    int dev = 0;
+    ReduceSlot r;
+    bool held = false, done = false;
+    hipStream_t s = nullptr;
+    int take(hipStream_t st)
+    {
+        s = st;
+        IMR_CHECK(current_device(&dev));
+        IMR_CHECK(reduce_slot_get(dev, &r));
+        held = true;
+        return IMRESH_OK;
+    }
+    ~SlotHold()
+    {
+        if (!held) return;
+        if (!done) hipStreamSynchronize(s);
+        reduce_slot_put(dev, r);
+    }
+};
 
 int vector_reduce(int op, const float *d, uint32_t n, float *out, void *stream)
 {
     IMR_CHECK(check_n(d, n));
     if (out == nullptr) return IMRESH_ERR_NULL;
     hipStream_t s = S(stream);
-    Scratch sc(s);
-    IMR_CHECK(sc.alloc(sizeof(float) * (kReduceBlocks + 64)));
-    float *part = (float *) sc.p, *res = part + kReduceBlocks;
+    SlotHold sh;
+    IMR_CHECK(sh.take(s));
+    float *part = (float *) sh.r.dev, *res = part + kReduceBlocks;
     IMR_CHECK(launch_reduce(op, d, n, part, kReduceBlocks, res, s));
-    return read_scalar(res, out, s);
+    IMR_HIP_CHECK(hipMemcpyAsync(sh.r.host, res, sizeof(float), hipMemcpyDeviceToHost, s));
+    IMR_HIP_CHECK(hipStreamSynchronize(s));
+    sh.done = true;
+    *out = sh.r.host[0];
+    return IMRESH_OK;
 }
 
 int hio_error(const float *g, const void *mask, int kind, uint32_t n, int invert, float *err,
@@ -657,16 +719,17 @@ int hio_error(const float *g, const void *mask, int kind, uint32_t n, int invert
 {
     if (g == nullptr || mask == nullptr || err == nullptr) return IMRESH_ERR_NULL;
     hipStream_t s = S(stream);
-    Scratch sc(s);
-    IMR_CHECK(sc.alloc(sizeof(float) * (2 * kReduceBlocks + 64)));
-    float *perr = (float *) sc.p;
+    SlotHold sh;
+    IMR_CHECK(sh.take(s));
+    float *perr = (float *) sh.r.dev;
     unsigned *pcnt = (unsigned *) (perr + kReduceBlocks);
     float *res = perr + 2 * kReduceBlocks;
     IMR_CHECK(launch_hio_error((const float2 *) g, mask, kind, n, invert, perr, pcnt, kReduceBlocks,
                                res, s));
-    float h[3];
-    IMR_HIP_CHECK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+    IMR_HIP_CHECK(hipMemcpyAsync(sh.r.host, res, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
     IMR_HIP_CHECK(hipStreamSynchronize(s));
+    sh.done = true;
+    const float *h = sh.r.host;
     *err = h[0];
     if (tot) *tot = h[1];
     if (nm) *nm = h[2];

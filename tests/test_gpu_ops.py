@@ -652,3 +652,46 @@ def test_ops_bit_exact_vs_reference_sources():
     assert nm.value == rn.value == float(mask.sum())
     for t in (tot.value, rt.value):
         assert abs(t - exact_tot) <= 1e-6 * exact_tot
+
+
+def test_reductions_concurrent_threads():
+    """Four host threads on their own streams call the synchronous
+    reductions and the HIO error at once, repeatedly (their scratch slots
+    are pooled per device); every result equals the single-threaded one."""
+    import threading
+    rng = np.random.default_rng(44)
+    n = 200003
+    x = (rng.standard_normal(n) * 10).astype(np.float32)
+    c = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    mask = (rng.random(n) < 0.5).astype(np.float32)
+    dx, dc, dm = B.from_array(x), B.from_array(c), B.from_array(mask)
+    exp = (_reduce("imresh_vector_max", x), _reduce("imresh_vector_min", x), _reduce("imresh_vector_sum", x))
+    e0, t0, n0 = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    im.check(L().imresh_calculate_hio_error(dc.ptr, dm.ptr, n, 0, _f(e0), _f(t0), _f(n0), None))
+    errors = []
+
+    def worker(k):
+        st = im.stream_create()
+        try:
+            for _ in range(25):
+                got = []
+                for name in ("imresh_vector_max", "imresh_vector_min", "imresh_vector_sum"):
+                    out = ctypes.c_float()
+                    im.check(getattr(L(), name)(dx.ptr, n, _f(out), st), name)
+                    got.append(np.float32(out.value))
+                e, t, m = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+                im.check(L().imresh_calculate_hio_error(dc.ptr, dm.ptr, n, 0, _f(e), _f(t), _f(m), st))
+                if tuple(got) != exp or (e.value, t.value, m.value) != (e0.value, t0.value, n0.value):
+                    errors.append((k, got, e.value))
+        except Exception as ex:   # reported below, from the main thread
+            errors.append((k, repr(ex)))
+        finally:
+            im.stream_destroy(st)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors[:4]
