@@ -145,6 +145,8 @@ _SIGS = {
     "imresh_read_txt": (c_int, [ctypes.c_char_p, _P, ctypes.c_size_t, POINTER(c_uint32), POINTER(c_uint32)]),
     "imresh_write_png": (c_int, [ctypes.c_char_p, _P, c_uint32, c_uint32]),
     "imresh_profile_forward_model": (c_int, [c_uint32, c_uint32, c_uint32, POINTER(c_float), _P]),
+    "imresh_debug_cycle_mask": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), c_uint32, _P, _P, _P,
+                                        POINTER(c_float), _P]),
 }
 
 
@@ -314,6 +316,24 @@ def shrink_wrap_batch_device(images: list, params: Params | None = None, stream=
                                                 res if want_results else None),
           "imresh_shrinkwrap_batch_device")
     return [res[i] for i in range(len(images))] if want_results else None
+
+
+def debug_cycle_mask(modulus: np.ndarray, cycle: int, params: Params | None = None, stream=None):
+    """The state of shrink-wrap cycle `cycle` of a reconstruction of
+    `modulus` as the fused kernels hold it: (g' the mask is built from,
+    complex64 H x W; mask, uint8 H x W, 1 = outside; blurred |g'|, float32;
+    threshold cut_c * max)."""
+    m = np.ascontiguousarray(modulus, dtype=np.float32)
+    h, w = m.shape
+    p = params if params is not None else default_params()
+    g = np.empty((h, w), np.complex64)
+    mask = np.empty((h, w), np.uint8)
+    blurred = np.empty((h, w), np.float32)
+    thr = c_float()
+    check(_lib().imresh_debug_cycle_mask(m.ctypes.data, w, h, ctypes.byref(p), cycle, g.ctypes.data,
+                                         mask.ctypes.data, blurred.ctypes.data, ctypes.byref(thr),
+                                         stream), "imresh_debug_cycle_mask")
+    return g, mask, blurred, thr.value
 
 
 def hybrid_input_output(io: np.ndarray, mask: np.ndarray, n_cycles: int = 0,

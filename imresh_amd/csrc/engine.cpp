@@ -44,14 +44,9 @@ inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 namespace imr {
 
-int ws_alloc(int device, int W, int H, Workspace **out)
+size_t ws_layout(Workspace *w, int W, int H, void *base)
 {
-    Workspace *w = new Workspace();
-    w->device = device;
-    w->W = W;
-    w->H = H;
-    w->n = (size_t) W * H;
-    const size_t n = w->n;
+    const size_t n = (size_t) W * H;
     const int nblur = blur_grid(W, H);
     const int nrow = H;   // >= number of row workgroups
     size_t off = 0;
@@ -66,13 +61,13 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     const size_t oPcnt = off; off = align256(off + 4 * (size_t) nrow);
     const size_t oScal = off; off = align256(off + 64);
     const size_t oFlags = off; off = align256(off + 64);
+    if (w == nullptr) return off;
+    w->W = W;
+    w->H = H;
+    w->n = n;
     w->bytes = off;
-    hipError_t e = hipMalloc(&w->base, off);
-    if (e != hipSuccess) {
-        delete w;
-        return from_hip(e);
-    }
-    char *b = (char *) w->base;
+    w->base = base;
+    char *b = (char *) base;
     w->X = (float2 *) (b + oX);
     w->G = (float2 *) (b + oG);
     w->Bn = (float *) (b + oBn);
@@ -84,6 +79,20 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     w->pcnt = (unsigned *) (b + oPcnt);
     w->scal = (float *) (b + oScal);
     w->flags = (int *) (b + oFlags);
+    return off;
+}
+
+int ws_alloc(int device, int W, int H, Workspace **out)
+{
+    Workspace *w = new Workspace();
+    w->device = device;
+    void *base = nullptr;
+    hipError_t e = hipMalloc(&base, ws_layout(nullptr, W, H, nullptr));
+    if (e != hipSuccess) {
+        delete w;
+        return from_hip(e);
+    }
+    ws_layout(w, W, H, base);
     e = hipHostMalloc((void **) &w->host_stage, 4096, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&w->ready, hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -450,8 +459,6 @@ int recon_begin(Workspace *w, const float *M, const Params &q, hipStream_t s)
     return launch_row(ROW_INIT, p, row_args(w, q), s);
 }
 
-namespace {
-
 // one shrink-wrap cycle on the workspace (the batch kernels with one image):
 // blur of |g'| with the cycle's weights row, threshold, n_hio fused HIO
 // iterations, closing inverse rows, error and early-exit flag
@@ -479,6 +486,8 @@ int enqueue_cycle(Workspace *w, const Params &q, const Plan2D &p, hipStream_t s)
     IMR_CHECK(launch_row(ROW_LAST, p, ra, s));
     return launch_err_finalize_batch(w->perr, w->pcnt, row_grid(p), q.target, w->scal, w->flags, 1, s);
 }
+
+namespace {
 
 std::mutex g_cap_mtx;
 std::map<int, hipStream_t> g_cap_streams;
@@ -512,6 +521,10 @@ int cycle_graph(Workspace *w, const Params &q, const Plan2D &p)
 {
     if (same_key(w, q, p)) return IMRESH_OK;
     if (w->gexec) {
+        // the previous user of the workspace may have replays of this graph
+        // still queued on its own stream (ws_checkout orders only the new
+        // stream behind them): wait for them before the graph is destroyed
+        if (w->pending) IMR_HIP_CHECK(hipEventSynchronize(w->ready));
         hipGraphExecDestroy(w->gexec);
         w->gexec = nullptr;
     }
@@ -672,6 +685,30 @@ void reduce_slot_put(int dev, const ReduceSlot &r)
 {
     std::lock_guard<std::mutex> lk(g_rs_mtx);
     g_rs_free[dev].push_back(r);
+}
+
+// idle slots (their calls have synchronized, so no kernel writes them) are
+// freed by imresh_release_cached; slots held by running calls stay
+void reduce_slots_release()
+{
+    std::map<int, std::vector<ReduceSlot>> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_rs_mtx);
+        drop.swap(g_rs_free);
+    }
+    for (auto &kv : drop)
+        for (const ReduceSlot &r : kv.second) {
+            hipFree(r.dev);
+            hipHostFree(r.host);
+        }
+}
+
+size_t reduce_slots_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_rs_mtx);
+    size_t k = 0;
+    for (auto &kv : g_rs_free) k += kv.second.size();
+    return k * kReduceSlotBytes;
 }
 
 // a slot held for one call and returned on every exit path; a call that
@@ -969,6 +1006,7 @@ int imresh_release_cached(void)
         trim_idle(0, drop);
     }
     for (Workspace *x : drop) ws_release(x);
+    reduce_slots_release();
     return guarded([] { return trim_idle_classes(0); });
 }
 
@@ -980,7 +1018,8 @@ int imresh_cached_bytes(uint64_t *bytes)
         std::lock_guard<std::mutex> lk(g_ws_mtx);
         ws = g_idle_bytes;
     }
-    *bytes = ws + idle_class_bytes();   // the two pools' locks are never held together
+    // the pools' locks are never held together
+    *bytes = ws + idle_class_bytes() + reduce_slots_bytes();
     return IMRESH_OK;
 }
 
@@ -1059,6 +1098,59 @@ int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
                       const imresh_params *params, void *stream, imresh_result *result)
 {
     return guarded([&] { return shrinkwrap_host_impl(io_host, width, height, params, stream, result); });
+}
+
+// Test hook: the reconstruction's own launches up to cycle `cycle`, with the
+// cycle's input g' copied out before the cycle and its mask, blurred |g'|
+// and threshold after it (the cycle's first row pass writes the mask; no
+// later pass of the cycle changes the mask or the blurred plane).
+static int debug_cycle_mask_impl(const float *modulus_host, uint32_t width, uint32_t height,
+                                 const imresh_params *params, uint32_t cycle, float *gprime_host,
+                                 uint8_t *mask_host, float *blurred_host, float *threshold_host,
+                                 void *stream)
+{
+    if (modulus_host == nullptr || gprime_host == nullptr || mask_host == nullptr) return IMRESH_ERR_NULL;
+    if (width == 0 || height == 0) return IMRESH_ERR_INVALID;
+    Params q;
+    IMR_CHECK(normalize(params, q));
+    if (cycle >= q.n_cycles) return IMRESH_ERR_INVALID;
+    int dev;
+    IMR_CHECK(current_device(&dev));
+    Plan2D p;
+    IMR_CHECK(get_plan2d(dev, width, height, &p));
+    hipStream_t s = S(stream);
+    Workspace *w;
+    IMR_CHECK(ws_checkout(dev, width, height, s, &w));
+    const size_t n = w->n;
+    int st = ws_tune(w, s);
+    if (st == IMRESH_OK)
+        st = from_hip(hipMemcpyAsync(w->M, modulus_host, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    if (st == IMRESH_OK) st = recon_begin(w, w->M, q, s);
+    for (unsigned c = 0; c < cycle && st == IMRESH_OK; ++c) st = recon_cycle(w, w->M, q, c, s);
+    if (st == IMRESH_OK)
+        st = from_hip(hipMemcpyAsync(gprime_host, w->X, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
+    if (st == IMRESH_OK) st = recon_cycle(w, w->M, q, cycle, s);
+    if (st == IMRESH_OK)
+        st = from_hip(hipMemcpyAsync(mask_host, w->mask, n, hipMemcpyDeviceToHost, s));
+    if (st == IMRESH_OK && blurred_host != nullptr)
+        st = from_hip(hipMemcpyAsync(blurred_host, w->B2, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (st == IMRESH_OK) st = recon_stage_results(w, s);
+    if (st == IMRESH_OK) st = from_hip(hipStreamSynchronize(s));
+    if (st == IMRESH_OK && threshold_host != nullptr) *threshold_host = w->host_stage[0];
+    if (st != IMRESH_OK) hipStreamSynchronize(s);
+    ws_return(w, s);
+    return st;
+}
+
+int imresh_debug_cycle_mask(const float *modulus_host, uint32_t width, uint32_t height,
+                            const imresh_params *params, uint32_t cycle, float *gprime_host,
+                            uint8_t *mask_host, float *blurred_host, float *threshold_host,
+                            void *stream)
+{
+    return guarded([&] {
+        return debug_cycle_mask_impl(modulus_host, width, height, params, cycle, gprime_host, mask_host,
+                                     blurred_host, threshold_host, stream);
+    });
 }
 
 }  // extern "C"

@@ -56,6 +56,9 @@ struct Workspace {
     const float *g_weights = nullptr;
 };
 
+// carves the workspace planes of W x H out of `base` (w NULL: only sizes);
+// returns the bytes the layout needs
+size_t ws_layout(Workspace *w, int W, int H, void *base);
 int ws_alloc(int device, int W, int H, Workspace **out);
 // batch class buffers: free idle ones whose work has finished, least
 // recently used first, until the idle bytes fit `cap`; bytes held idle
@@ -85,6 +88,8 @@ extern std::atomic<bool> g_use_graphs;
 int ws_tune(Workspace *w, hipStream_t s);
 int recon_begin(Workspace *w, const float *M, const Params &q, hipStream_t s);
 int recon_cycle(Workspace *w, const float *M, const Params &q, unsigned c, hipStream_t s);
+// the launches of one cycle (what recon_cycle replays as a graph)
+int enqueue_cycle(Workspace *w, const Params &q, const Plan2D &p, hipStream_t s);
 int recon_end(Workspace *w, float *io_dev, hipStream_t s);
 int recon_stage_results(Workspace *w, hipStream_t s);
 void recon_read_staged(const Workspace *w, imresh_result *res, int *done);

@@ -97,9 +97,11 @@ int imresh_stream_sync(void *stream);
 enum { IMRESH_OPT_CYCLE_GRAPHS = 1, IMRESH_OPT_POOL_MIB = 2 };
 int imresh_set_option(int option, int value);
 /* Frees every idle pooled workspace and batch buffer whose work has
- * finished (all of them after the caller's streams are synchronized). */
+ * finished (all of them after the caller's streams are synchronized) and
+ * the idle scratch slots of the synchronous reductions. */
 int imresh_release_cached(void);
-/* Device bytes held by idle pooled workspaces and batch buffers. */
+/* Device bytes held by idle pooled workspaces, batch buffers and reduction
+ * scratch slots. */
 int imresh_cached_bytes(uint64_t *bytes);
 
 /* A non-blocking HIP stream at the default priority (for callers without a
@@ -123,6 +125,20 @@ int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
  * stream and reads the cycle count / error back. */
 int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
                              const imresh_params *params, void *stream, imresh_result *result);
+
+/* Test hook (no reference counterpart; it exposes the state of the
+ * reference's mask update, shrinkWrap.cpp:191-242 / cudaShrinkWrap.cu:
+ * 237-277, as the fused kernels hold it): runs the reconstruction of the host
+ * modulus with `params` through cycle `cycle` (< n_cycles) and returns
+ *   gprime_host   : g' the cycle's mask is built from (W*H complex, 8 B each),
+ *   mask_host     : that cycle's support mask, uint8 (1 = outside),
+ *   blurred_host  : the blurred |g'| (W*H floats; may be NULL),
+ *   threshold_host: cut_c * max of the blurred plane (may be NULL).
+ * Synchronous. */
+int imresh_debug_cycle_mask(const float *modulus_host, uint32_t width, uint32_t height,
+                            const imresh_params *params, uint32_t cycle, float *gprime_host,
+                            uint8_t *mask_host, float *blurred_host, float *threshold_host,
+                            void *stream);
 
 /* ---- batched reconstruction (no reference counterpart: the reference runs
  * one image per cudaShrinkWrap call and overlaps calls only through its task
@@ -345,7 +361,10 @@ typedef void (*imresh_task_callback)(void *user, float *io_host, uint32_t width,
                                      uint32_t height, int status);
 /* taskQueueInit, taskQueue.cu:231-234 (streams per device: 0 = default) */
 int imresh_task_queue_init(int streams_per_device);
-/* taskQueueDeinit, taskQueue.cu:236-251: waits for all tasks */
+/* taskQueueDeinit, taskQueue.cu:236-251: waits for all tasks; producers
+ * still blocked in imresh_add_task return IMRESH_ERR_NOT_INITIALIZED.
+ * Called from a task callback (a worker thread, which it would have to
+ * join) it returns IMRESH_ERR_INVALID and leaves the queue running. */
 int imresh_task_queue_deinit(void);
 /* addTask, taskQueue.cu:147-193: blocks while all slots are busy */
 int imresh_add_task(float *io_host, uint32_t width, uint32_t height,
