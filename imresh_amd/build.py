@@ -80,19 +80,22 @@ def build(verbose: bool = False) -> str:
 
 
 EXAMPLE = os.path.join(LIBDIR, "imresh_queue_example")
+MINI_EXAMPLE = os.path.join(LIBDIR, "imresh_shrinkwrap_example")
 
 
 def _build_example(verbose: bool) -> None:
-    """The C++ drop-in example (examples/queue_png.cpp) against the built library."""
-    src = os.path.join(ROOT, "imresh_amd", "examples", "queue_png.cpp")
-    cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", EXAMPLE + ".tmp",
-           "-L", LIBDIR, "-limresh_hip", "-Wl,-rpath,$ORIGIN"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("example build failed:\n" + r.stdout + r.stderr)
-    os.replace(EXAMPLE + ".tmp", EXAMPLE)
+    """The C++ drop-in examples (examples/queue_png.cpp, examples/shrinkwrap_mini.cpp)
+    against the built library."""
+    for name, out in (("queue_png.cpp", EXAMPLE), ("shrinkwrap_mini.cpp", MINI_EXAMPLE)):
+        src = os.path.join(ROOT, "imresh_amd", "examples", name)
+        cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", out + ".tmp",
+               "-L", LIBDIR, "-limresh_hip", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed ({name}):\n" + r.stdout + r.stderr)
+        os.replace(out + ".tmp", out)
 
 
 def build_oracle(verbose: bool = False) -> None:

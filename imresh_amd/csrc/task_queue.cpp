@@ -52,9 +52,13 @@ struct Queue {
 } g_q;
 
 std::mutex g_init_mtx;
+// set on the queue's worker threads: a deinit from a task callback would
+// have to join its own thread
+thread_local bool t_queue_worker = false;
 
 void worker_main(size_t idx)
 {
+    t_queue_worker = true;
     const Slot &slot = g_q.slots[idx];
     hipSetDevice(slot.device);
     for (;;) {
@@ -113,12 +117,15 @@ int queue_init(int per_device)
 
 int queue_deinit()
 {
+    if (t_queue_worker) return IMRESH_ERR_INVALID;
     std::lock_guard<std::mutex> lk(g_init_mtx);
     if (!g_q.running) return IMRESH_OK;
     {
         std::lock_guard<std::mutex> l2(g_q.mtx);
         g_q.closing = true;
     }
+    // producers blocked in queue_add wake up and fail; workers drain the FIFO
+    g_q.cv_space.notify_all();
     g_q.cv_work.notify_all();
     for (auto &s : g_q.slots)
         if (s.worker.joinable()) s.worker.join();
@@ -153,7 +160,10 @@ int imresh_task_queue_init(int streams_per_device)
     return imr::guarded([&] { return queue_init(streams_per_device); });
 }
 
-int imresh_task_queue_deinit(void) { return queue_deinit(); }
+int imresh_task_queue_deinit(void)
+{
+    return imr::guarded([] { return queue_deinit(); });
+}
 
 int imresh_add_task(float *io_host, uint32_t width, uint32_t height, const imresh_params *params,
                     imresh_task_callback callback, void *user)

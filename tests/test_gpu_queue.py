@@ -79,3 +79,77 @@ def test_cpp_dropin_example(tmp_path):
     exp = im.read_image(ref)
     for i in range(3):
         assert np.array_equal(im.read_image(f"{prefix}{i}.png"), exp), i
+
+
+MINI = os.path.join(os.path.dirname(im.LIB_PATH), "imresh_shrinkwrap_example")
+
+
+def test_cpp_shrinkwrap_mini_example(tmp_path):
+    """examples/shrinkwrap_mini.cpp calls imresh::algorithms::shrinkWrap
+    through the C++ header as the reference's miniExample.cpp:93-116 does:
+    a null buffer returns 1, shrinkWrap(io, W, H, 64, 1e-6) equals
+    imresh_shrinkwrap with the reference defaults bit for bit, and passing 0
+    for every later parameter selects those defaults (shrinkWrap.cpp:133-140)."""
+    assert os.path.exists(MINI), "build first: __graft_entry__.build()"
+    obj, M = syn.workload(96, 80, n_shapes=6, central=40, index=23)
+    src = str(tmp_path / "modulus.f32")
+    M.astype(np.float32).tofile(src)
+    prefix = str(tmp_path / "out_")
+    r = subprocess.run([MINI, src, "96", "80", prefix, "6"], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.split()
+    assert lines[:6] == ["null", "1", "a", "0", "b", "0"], r.stdout
+    a = np.fromfile(prefix + "a.f32", np.float32).reshape(80, 96)
+    b = np.fromfile(prefix + "b.f32", np.float32).reshape(80, 96)
+    x = M.copy()
+    p = im.default_params(n_cycles=6, target_error=1e-6)
+    res = im.Result()
+    im.check(L().imresh_shrinkwrap(x.ctypes.data, 96, 80, p, None, res))
+    assert np.array_equal(a, x)
+    assert np.array_equal(b, x)
+    assert res.cycles_run >= 1
+
+
+def test_deinit_from_callback_is_refused():
+    """A task callback runs on a queue worker; a deinit there would join its
+    own thread, so it returns IMRESH_ERR_INVALID and the queue keeps
+    running."""
+    got = []
+
+    def write_out(io, w, h, name):
+        got.append(L().imresh_task_queue_deinit())
+
+    M = syn.workload(64, 64, n_shapes=4, central=32, index=24)[1]
+    im.task_queue_init()
+    try:
+        im.add_task(M.copy(), write_out, "t", n_cycles=1, n_hio_cycles=2)
+    finally:
+        im.task_queue_deinit()
+    assert got == [2]
+
+
+def test_deinit_wakes_every_blocked_producer():
+    """With one slot and more producers blocked in addTask than the FIFO
+    holds, deinit must wake all of them (each returns OK or
+    IMRESH_ERR_NOT_INITIALIZED) instead of leaving some asleep."""
+    M = syn.workload(256, 256, n_shapes=8, central=128, index=25)[1]
+    p = im.default_params(im.DEFAULTS_QUEUE, n_cycles=4, n_hio_cycles=20)
+    im.check(L().imresh_task_queue_init(1))
+    status, lock = [], threading.Lock()
+    bufs = [M.copy() for _ in range(8)]
+
+    def producer(b):
+        st = L().imresh_add_task(b.ctypes.data, 256, 256, p, im.TASK_CALLBACK(), None)
+        with lock:
+            status.append(st)
+
+    threads = [threading.Thread(target=producer, args=(b,), daemon=True) for b in bufs]
+    for t in threads:
+        t.start()
+    import time
+    time.sleep(0.05)
+    im.check(L().imresh_task_queue_deinit())
+    for t in threads:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in threads), "a producer stayed blocked after deinit"
+    assert len(status) == 8 and set(status) <= {0, 8}, status
