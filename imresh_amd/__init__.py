@@ -145,6 +145,8 @@ _SIGS = {
     "imresh_read_txt": (c_int, [ctypes.c_char_p, _P, ctypes.c_size_t, POINTER(c_uint32), POINTER(c_uint32)]),
     "imresh_write_png": (c_int, [ctypes.c_char_p, _P, c_uint32, c_uint32]),
     "imresh_profile_forward_model": (c_int, [c_uint32, c_uint32, c_uint32, POINTER(c_float), _P]),
+    "imresh_shrinkwrap_device_noalloc": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), _P,
+                                                 POINTER(Result)]),
     "imresh_debug_cycle_mask": (c_int, [_P, c_uint32, c_uint32, POINTER(Params), c_uint32, _P, _P, _P,
                                         POINTER(c_float), _P]),
 }

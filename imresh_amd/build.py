@@ -26,7 +26,8 @@ NUM_PLAN_GROUPS = 7   # IMR_NUM_PLAN_GROUPS in csrc/plans.hpp
 UNITS = ([("k_row.hip", f"k_row_g{g}", [f"-DIMR_GROUP={g}"]) for g in range(NUM_PLAN_GROUPS)]
          + [("k_col.hip", f"k_col_g{g}", [f"-DIMR_GROUP={g}"]) for g in range(NUM_PLAN_GROUPS)]
          + [(s, s.rsplit(".", 1)[0], []) for s in ("k_misc.hip", "k_poisson.hip", "k_step8.hip", "plan.cpp", "engine.cpp",
-                                                    "batch.cpp", "serve.cpp", "task_queue.cpp", "io.cpp")])
+                                                    "batch.cpp", "serve.cpp", "task_queue.cpp", "io.cpp",
+                                                    "noalloc.cpp")])
 
 
 def hipcc() -> str:

@@ -919,6 +919,7 @@ const char *imresh_strerror(int status)
     case IMRESH_ERR_BUDGET: return "request exceeds the HBM budget";
     case IMRESH_ERR_IO: return "file unreadable, unwritable or of an unsupported format";
     case IMRESH_ERR_INTERNAL: return "internal host-side failure";
+    case IMRESH_ERR_TOO_LARGE: return "exceeds the fixed scratch of the allocation-free entry point";
     default: return "unknown status";
     }
 }

@@ -167,6 +167,9 @@ bool get_tuned_variants(int device, int W, int H, int *rowVar, int *colVar);
 
 int get_line_plan(int device, int n, LinePlan *out);
 int get_plan2d(int device, int W, int H, Plan2D *out);
+// the plan of W x H if both line plans already exist on the device (no
+// allocation; IMRESH_ERR_NOT_INITIALIZED otherwise)
+int find_plan2d(int device, int W, int H, Plan2D *out);
 bool line_supported(int n);
 
 int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s);
@@ -225,6 +228,10 @@ int launch_hio_error(const float2 *g, const void *mask, int mask_kind, size_t n,
 int launch_pack_mask(const uint8_t *mask, size_t n, uint32_t *bits, hipStream_t s);
 int launch_fill_state(float2 *X, float2 *G, uint8_t *mask, float *M, float *Bn, size_t n,
                       int W, hipStream_t s);
+// n floats from host memory (src NULL: zeros) to device memory through
+// kernel arguments, kStoreWords per launch: no staging buffer, capturable
+constexpr int kStoreWords = 480;
+int launch_store_words(float *dst, const float *src, size_t n, hipStream_t s);
 
 constexpr int kReduceBlocks = 512;
 constexpr int kBlurTX = 64;

@@ -361,6 +361,18 @@ __global__ void k_fill_state(float2 *X, float2 *G, uint8_t *mask, float *M, floa
     }
 }
 
+// up to kStoreWords host values carried in the kernel arguments (copied at
+// launch, so the host array may be reused at once) stored to device memory:
+// uploads that need no staging allocation and can be captured in a graph
+struct StoreWords {
+    float v[kStoreWords];
+};
+
+__global__ void k_store_words(float *dst, unsigned n, StoreWords a)
+{
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) dst[i] = a.v[i];
+}
+
 // ------------------------------------------------------------ launchers
 static int grid_1d(size_t n)
 {
@@ -694,6 +706,18 @@ int launch_pack_mask(const uint8_t *mask, size_t n, uint32_t *bits, hipStream_t 
     if (nw == 0) return IMRESH_OK;
     hipLaunchKernelGGL(k_pack_mask, dim3(grid_1d(nw)), dim3(256), 0, s, mask, n, bits);
     return from_hip(hipGetLastError());
+}
+
+int launch_store_words(float *dst, const float *src, size_t n, hipStream_t s)
+{
+    for (size_t off = 0; off < n; off += kStoreWords) {
+        StoreWords a;
+        const unsigned k = (unsigned) std::min<size_t>(kStoreWords, n - off);
+        for (unsigned i = 0; i < kStoreWords; ++i) a.v[i] = i < k ? (src ? src[off + i] : 0.f) : 0.f;
+        hipLaunchKernelGGL(k_store_words, dim3(1), dim3(256), 0, s, dst + off, k, a);
+        IMR_HIP_CHECK(hipGetLastError());
+    }
+    return IMRESH_OK;
 }
 
 int launch_fill_state(float2 *X, float2 *G, uint8_t *mask, float *M, float *Bn, size_t n, int W,

@@ -264,6 +264,24 @@ int get_plan2d(int device, int W, int H, Plan2D *out)
     return IMRESH_OK;
 }
 
+int find_plan2d(int device, int W, int H, Plan2D *out)
+{
+    if (W < 1 || H < 1) return IMRESH_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(g_plan_mtx);
+    auto r = g_lines.find({device, W}), c = g_lines.find({device, H});
+    if (r == g_lines.end() || c == g_lines.end()) return IMRESH_ERR_NOT_INITIALIZED;
+    Plan2D p{};
+    p.W = W;
+    p.H = H;
+    p.row = r->second;
+    p.col = c->second;
+    auto t = g_tuned.find({device, W, H});
+    p.rowVar = t != g_tuned.end() ? t->second.first : 0;
+    p.colVar = t != g_tuned.end() ? t->second.second : 0;
+    *out = p;
+    return IMRESH_OK;
+}
+
 int row_grid(const Plan2D &p)
 {
     int L = 1;

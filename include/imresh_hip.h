@@ -38,7 +38,8 @@ enum imresh_status {
     IMRESH_ERR_NOT_INITIALIZED = 8,  /* addTask before taskQueueInit (taskQueue.cu:163) */
     IMRESH_ERR_BUDGET = 9,           /* request exceeds the configured HBM budget */
     IMRESH_ERR_IO = 10,              /* file unreadable / unwritable or format unsupported */
-    IMRESH_ERR_INTERNAL = 11         /* unexpected host-side failure (e.g. thread creation) */
+    IMRESH_ERR_INTERNAL = 11,        /* unexpected host-side failure (e.g. thread creation) */
+    IMRESH_ERR_TOO_LARGE = 12        /* beyond the fixed scratch of the allocation-free entry point */
 };
 
 /* Default-substitution rule for parameters <= 0 (or == 0 for counts). */
@@ -125,6 +126,28 @@ int imresh_shrinkwrap(float *io_host, uint32_t width, uint32_t height,
  * stream and reads the cycle count / error back. */
 int imresh_shrinkwrap_device(float *io_dev, uint32_t width, uint32_t height,
                              const imresh_params *params, void *stream, imresh_result *result);
+
+/* Allocation-free variant of imresh_shrinkwrap_device.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The reconstruction's state lives in a
+ * fixed device array of the library (one per GPU, held until the process
+ * ends) instead of a pooled workspace: the call allocates nothing, records
+ * no memset and launches each cycle's kernels directly.  Limits: the
+ * workspace of W x H (29 W*H bytes plus partial sums) must fit in
+ * IMRESH_NOALLOC_MAX_BYTES (2048 x 2048 fits) and n_cycles must not exceed
+ * IMRESH_NOALLOC_MAX_CYCLES, else IMRESH_ERR_TOO_LARGE.  The FFT tables of
+ * the size must already exist on the device (built by any earlier call of
+ * the size, e.g. imresh_autotune), else IMRESH_ERR_NOT_INITIALIZED.  Every
+ * call on one GPU uses that GPU's single array: calls that overlap on one
+ * GPU (other streams or threads) race.  The result equals
+ * imresh_shrinkwrap_device's bit for bit.  Asynchronous when result ==
+ * NULL (the only form usable under capture); otherwise synchronizes. */
+#define IMRESH_NOALLOC_MAX_CYCLES 256
+#define IMRESH_NOALLOC_MAX_BYTES (134217728ull - 65536ull)
+int imresh_shrinkwrap_device_noalloc(float *io_dev, uint32_t width, uint32_t height,
+                                     const imresh_params *params, void *stream,
+                                     imresh_result *result);
 
 /* Test hook (no reference counterpart; it exposes the state of the
  * reference's mask update, shrinkWrap.cpp:191-242 / cudaShrinkWrap.cu:
