@@ -114,6 +114,58 @@ def cpu_baseline(M: np.ndarray, M256: np.ndarray, M8=None, budget_s: float = 12.
     return out
 
 
+def parity_check(L, M: np.ndarray) -> dict:
+    """The stated errors of the 2048^2 workload's fused kernels against the
+    CPU oracle (the checker), measured in this run:
+      * the support mask the fused path builds in cycles 0 and 1 (blur, tile
+        maxima, threshold, cut-off fused into the first row pass) against the
+        oracle's mask update of the same g': elements equal (bit-exact bar);
+      * one fused HIO iteration from that state against the float64 oracle:
+        relative L2 (bar 1e-5), with the float32 oracle's own distance;
+      * a full 20 x 20 reconstruction of a converging 256^2 workload against
+        the float64 oracle's end result (tests/golden/converging.npz): NCC to
+        the object (bar 1e-3), the object's relative L2 (bar 1e-3) and the
+        HIO error."""
+    import oracle  # CPU checker; only the baseline legs use it
+
+    oracle.set_fft_threads(min(16, os.cpu_count() or 8))
+    out = {"tolerances": {"mask": "bit-exact", "one_iteration_rel_l2": 1e-5, "ncc": 1e-3,
+                          "object_rel_l2": 1e-3}}
+    p = im.default_params(n_cycles=2, target_error=FLT_MIN, n_hio_cycles=N_HIO)
+    masks = {}
+    for c in (0, 1):
+        g, mask, blurred, thr = im.debug_cycle_mask(M, c, p)
+        sigma = float(oracle.sigma_schedule(3.0, 0.01, 2)[c])
+        exp = oracle.make_mask(g, sigma, 0.04 if c == 0 else 0.2, np.float32).astype(np.uint8)
+        masks[c] = (g, mask)
+        out[f"cycle{c}_mask_elements_equal"] = int(np.sum(mask == exp))
+        out[f"cycle{c}_mask_elements"] = int(mask.size)
+        out[f"cycle{c}_masked"] = int(mask.sum())
+    g, mask = masks[0]
+    dg, dgp = im.DeviceBuffer.from_array(g), im.DeviceBuffer.from_array(g)
+    dm, dM = im.DeviceBuffer.from_array(mask), im.DeviceBuffer.from_array(M)
+    im.check(L.imresh_hio_iterations(dg.ptr, dgp.ptr, dm.ptr, dM.ptr, W, H, np.float32(0.9), 1, None))
+    g_gpu = dg.download(np.complex64, M.shape)
+    g64, _ = oracle.hio_iterations(g.astype(np.complex128), g.astype(np.complex128), mask.astype(np.float64),
+                                   M.astype(np.float64), 0.9, 1, np.float64)
+    g32, _ = oracle.hio_iterations(g, g.copy(), mask.astype(np.float32), M, 0.9, 1, np.float32)
+    out["one_iteration_rel_l2_vs_fp64"] = float(f"{syn.rel_l2(g_gpu, g64):.3e}")
+    out["one_iteration_rel_l2_fp32_oracle_vs_fp64"] = float(f"{syn.rel_l2(g32, g64):.3e}")
+    z = np.load(os.path.join(ROOT, "tests", "golden", "converging.npz"), allow_pickle=False)
+    n, shapes, central, idx = (int(v) for v in z["c256a_spec"])
+    obj, Mc = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
+    x = Mc.copy()
+    r = im.shrink_wrap(x, n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
+    y0, x0, h, w = (int(v) for v in z["c256a_box"])
+    ref = z["c256a_crop64"].astype(np.float64)
+    out["converging_256"] = {
+        "ncc_gpu": round(syn.ncc(obj, x), 9), "ncc_fp64": round(float(z["c256a_ncc64"]), 9),
+        "object_rel_l2_vs_fp64": float(f"{np.linalg.norm(x[y0:y0 + h, x0:x0 + w] - ref) / np.linalg.norm(ref):.3e}"),
+        "hio_error_gpu": float(r.last_error), "hio_error_fp64": float(z["c256a_err64"]),
+        "hio_error_fp32_oracle": float(z["c256a_err32"]), "hio_error_first_cycle_fp64": float(z["c256a_err64_c1"])}
+    return out
+
+
 def _time_ms(fn) -> float:
     t = time.perf_counter()
     fn()
@@ -712,9 +764,12 @@ def main() -> int:
         fftc = [fft_compare.compare(s_) for s_ in (2048, 8192)]
 
     cpu = None
+    parity = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
         cpu = cpu_baseline(M, M256, M8 if args.cpu_large else None)
+        log("[bench] parity against the oracle")
+        parity = parity_check(L, M)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "HIO iters/s", "n_gpus": 1,
@@ -725,7 +780,7 @@ def main() -> int:
                                "sigma0 3.0, cut 0.2/0.04, 64 shapes in central 1024^2",
                    "iterations_per_step": N_CYCLES * N_HIO, "width": W, "height": H,
                    "row_variant": rv.value, "col_variant": cv.value},
-        "roofline": roofline, "cpu_baseline": cpu, "quality": quality, "small_256": small,
+        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "quality": quality, "small_256": small,
         "batched": batch,
         "serving": serve, "large_8192": large, "gpu_per_op_ms": per_op, "fft_vs_rocfft": fftc,
         "forward_model": fwd,

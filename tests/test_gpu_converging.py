@@ -1,0 +1,73 @@
+"""Full 20 x 20 reconstructions against the float64 oracle's end results
+(GPU).
+
+tests/golden/converging.npz (make_converging.py) holds the float64 oracle's
+end results of three workloads of the benchmark's object generator (256^2
+with one and two shapes, 2048^2 with two shapes, no Poisson noise) on which
+shrink-wrap converges to the known object in every precision, so the end
+result no longer depends on rounding.  The GPU reconstruction of the same
+modulus must match it:
+
+* NCC to the known object within 1e-3 of the float64 oracle's (the
+  benchmark definition's tolerance; both are 1.000000 here);
+* the reconstruction itself within 1e-3 relative L2 of the float64 end
+  result (inside the object's box; outside it the residual stays as small
+  as the oracle's);
+* the last cycle's masked count equal to the float64 oracle's;
+* the final HIO error sqrtf(sum_out |g'|) / n_out: at convergence it
+  measures a residual outside the support of ~1e-5 of the reconstruction's
+  L1 mass, which float32 rounding of the transforms decides -- the float32
+  restatement of the reference lands 1-18% away from float64 on these
+  workloads (at 2048^2, float32 runs on inputs perturbed by 1e-6 land
+  9-11% away), so
+  the error itself is held to 25% of the float64 value, and the residual's
+  share of the L1 mass, (e n_out)^2 / sum |Re g'|, to 1e-3 absolute.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import imresh_amd as im
+from helpers import FLT_MIN
+from imresh_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+FIX = os.path.join(os.path.dirname(__file__), "golden", "converging.npz")
+NAMES = ["c256a", "c256b", "c2048"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_full_reconstruction_matches_float64_oracle(name):
+    z = np.load(FIX, allow_pickle=False)
+    n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
+    obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
+    x = M.copy()
+    r = im.shrink_wrap(x, n_cycles=20, target_error=FLT_MIN, n_hio_cycles=20)
+    assert r.cycles_run == 20
+    e64, e32, e1 = float(z[f"{name}_err64"]), float(z[f"{name}_err32"]), float(z[f"{name}_err64_c1"])
+    n64 = float(z[f"{name}_ncc64"])
+    ncc = syn.ncc(obj, x)
+    y0, x0, h, w = (int(v) for v in z[f"{name}_box"])
+    crop = x[y0:y0 + h, x0:x0 + w].astype(np.float64)
+    ref = z[f"{name}_crop64"].astype(np.float64)
+    rel = np.linalg.norm(crop - ref) / np.linalg.norm(ref)
+    inside = np.zeros(x.shape, bool)
+    inside[y0:y0 + h, x0:x0 + w] = True
+    out_gpu = float(np.linalg.norm(x[~inside].astype(np.float64)))
+    out64, norm64 = float(z[f"{name}_out64"]), float(z[f"{name}_norm64"])
+    masked64 = float(z[f"{name}_masked"])
+    l1 = float(np.abs(x.astype(np.float64)).sum())
+    share_gpu = (r.last_error * r.last_masked) ** 2 / l1
+    share64 = (e64 * masked64) ** 2 / l1
+    print(f"{name} ({n}^2, {shapes} shapes): NCC gpu {ncc:.9f} fp64 {n64:.9f}; object rel L2 vs fp64 "
+          f"{rel:.2e}; outside-box norm gpu {out_gpu / norm64:.2e} fp64 {out64 / norm64:.2e} (of the "
+          f"image norm); HIO error gpu {r.last_error:.7g} fp64 {e64:.7g} fp32 {e32:.7g} first cycle "
+          f"{e1:.5g}; residual share of the L1 mass gpu {share_gpu:.3e} fp64 {share64:.3e}; masked "
+          f"{r.last_masked:.0f}/{masked64:.0f}")
+    assert abs(ncc - n64) <= 1e-3
+    assert rel <= 1e-3
+    assert out_gpu <= max(2 * out64, 1e-3 * norm64)
+    assert r.last_masked == masked64
+    assert abs(r.last_error - e64) <= 0.25 * e64
+    assert abs(share_gpu - share64) <= 1e-3

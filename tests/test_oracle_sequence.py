@@ -197,3 +197,27 @@ def test_hio_iteration_outside_support_closed_form():
     g1, gp1 = oracle.hio_iterations(g, gp, mask, M, 0.9, 1, np.float64)
     assert np.max(np.abs(gp1 - exp_gp)) <= 1e-15 * np.abs(exp_gp).max() * 4
     assert np.max(np.abs(g1 - g.size * exp_gp)) <= 1e-12 * g.size
+
+
+def test_converging_fixture_float32_within_the_gpu_bounds():
+    """tests/golden/converging.npz (make_converging.py): the float64 oracle
+    reproduces its 256^2 end result, and the float32 oracle -- the
+    reference's own precision -- meets the bounds the GPU test applies
+    (NCC within 1e-3, the reconstruction within 1e-3 relative L2)."""
+    import os
+    from imresh_amd import synthetic as syn
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "converging.npz"), allow_pickle=False)
+    oracle.set_fft_threads(8)
+    n, shapes, central, idx = (int(v) for v in z["c256a_spec"])
+    obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
+    kw = dict(n_cycles=20, target_error=np.finfo(np.float32).tiny, n_hio_cycles=20)
+    x64, s64 = oracle.shrink_wrap(M, dtype=np.float64, **kw)
+    y0, x0, h, w = (int(v) for v in z["c256a_box"])
+    assert np.array_equal(x64[y0:y0 + h, x0:x0 + w].astype(np.float32), z["c256a_crop64"])
+    assert s64.last_error == np.float32(z["c256a_err64"])
+    x32, s32 = oracle.shrink_wrap(M, dtype=np.float32, **kw)
+    ref = z["c256a_crop64"].astype(np.float64)
+    rel = np.linalg.norm(x32[y0:y0 + h, x0:x0 + w] - ref) / np.linalg.norm(ref)
+    assert rel <= 1e-3
+    assert abs(syn.ncc(obj, x32) - float(z["c256a_ncc64"])) <= 1e-3
+    assert abs(s32.last_error - float(z["c256a_err64"])) <= 1e-3 * float(z["c256a_err64_c1"])
