@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "imresh_amd", "csrc")
 LIBDIR = os.path.join(ROOT, "imresh_amd", "lib")
 LIBNAME = "libimresh_hip.so"
-NUM_PLAN_GROUPS = 7   # IMR_NUM_PLAN_GROUPS in csrc/plans.hpp
+NUM_PLAN_GROUPS = 8   # IMR_NUM_PLAN_GROUPS in csrc/plans.hpp
 # (source, object stem, extra defines): the row / column kernels are compiled
 # once per plan group so the instantiations build in parallel
 UNITS = ([("k_row.hip", f"k_row_g{g}", [f"-DIMR_GROUP={g}"]) for g in range(NUM_PLAN_GROUPS)]

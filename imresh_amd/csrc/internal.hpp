@@ -150,7 +150,10 @@ constexpr int kFourStepMinLine = 8192;
 constexpr int kBlueVariant = 1000;
 constexpr int kBlueColDefault = 1;
 constexpr int kBlueRowDefault = 2;
-constexpr int kMaxLine = 4096;   // longest line a Bluestein plan serves
+// longest line length: lengths without a compiled plan up to it run the
+// Bluestein kernels of a power-of-two convolution length up to 2 kMaxLine;
+// plans longer than kMaxLine exist only as such convolution lengths
+constexpr int kMaxLine = 8192;
 
 struct Plan2D {
     int W, H;
@@ -188,6 +191,7 @@ IMR_DECLARE_GROUP(3)
 IMR_DECLARE_GROUP(4)
 IMR_DECLARE_GROUP(5)
 IMR_DECLARE_GROUP(6)
+IMR_DECLARE_GROUP(7)
 // number of row workgroups of the plan's row variant
 int row_grid(const Plan2D &p);
 int blur_grid(int W, int H);

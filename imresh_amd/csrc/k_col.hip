@@ -179,6 +179,9 @@ int launch_pf(bool pf, int mode, ColArgs a, hipStream_t s)
 }
 
 template <int N, int... Rs>
+int launch_col_direct(int var, int mode, ColArgs a, hipStream_t s);
+
+template <int N, int... Rs>
 int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
 {
     using P16 = PlanE<16, N, Rs...>;
@@ -194,6 +197,17 @@ int launch_col_variant(int var, int mode, ColArgs a, hipStream_t s)
         default: break;
         }
     }
+    if constexpr (N > kMaxLine) {
+        return IMRESH_ERR_UNSUPPORTED_SIZE;   // a convolution length only
+    } else {
+        return launch_col_direct<N, Rs...>(var, mode, a, s);
+    }
+}
+
+template <int N, int... Rs>
+int launch_col_direct(int var, int mode, ColArgs a, hipStream_t s)
+{
+    using P16 = PlanE<16, N, Rs...>;
     constexpr int ns = col_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;

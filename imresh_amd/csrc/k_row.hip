@@ -233,6 +233,9 @@ int launch_pf(bool pf, int mode, RowArgs a, hipStream_t s)
 }
 
 template <int N, int... Rs>
+int launch_row_direct(int var, int mode, RowArgs a, hipStream_t s);
+
+template <int N, int... Rs>
 int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
 {
     using P16 = PlanE<16, N, Rs...>;
@@ -242,6 +245,17 @@ int launch_row_variant(int var, int mode, RowArgs a, hipStream_t s)
         if (var == kBlueVariant + 2)
             return launch_variant<P16, (P16::L > 1 ? P16::L / 2 : 1), false, true>(mode, a, s);
     }
+    if constexpr (N > kMaxLine) {
+        return IMRESH_ERR_UNSUPPORTED_SIZE;   // a convolution length only
+    } else {
+        return launch_row_direct<N, Rs...>(var, mode, a, s);
+    }
+}
+
+template <int N, int... Rs>
+int launch_row_direct(int var, int mode, RowArgs a, hipStream_t s)
+{
+    using P16 = PlanE<16, N, Rs...>;
     constexpr int ns = row_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;

@@ -149,7 +149,7 @@ std::vector<float2> blue_tables(int n, int m)
 
 }  // namespace
 
-bool line_supported(int n) { return find_info(n) != nullptr || (n >= 1 && n <= kMaxLine); }
+bool line_supported(int n) { return n >= 1 && n <= kMaxLine; }
 
 int get_line_plan(int device, int n, LinePlan *out)
 {
@@ -205,7 +205,7 @@ int get_line_plan(int device, int n, LinePlan *out)
     lp.L = pi->L;
     lp.C = pi->C;
     lp.tw = dev;
-    if (n >= kFourStepMinLine && n % 8 == 0 && find_info(n / 8) != nullptr) {
+    if (n >= kFourStepMinLine && n <= kMaxLine && n % 8 == 0 && find_info(n / 8) != nullptr) {
         // four-step column pass: its length-n/8 plan (created now: the pass
         // may first run inside a graph capture, where no allocation or copy
         // may happen) and the twiddles exp(-2 pi i j / n) of its outer steps
@@ -254,6 +254,7 @@ bool get_tuned_variants(int device, int W, int H, int *rowVar, int *colVar)
 int get_plan2d(int device, int W, int H, Plan2D *out)
 {
     if (W < 1 || H < 1) return IMRESH_ERR_INVALID;
+    if (!line_supported(W) || !line_supported(H)) return IMRESH_ERR_UNSUPPORTED_SIZE;
     Plan2D p{};
     p.W = W;
     p.H = H;
@@ -267,6 +268,7 @@ int get_plan2d(int device, int W, int H, Plan2D *out)
 int find_plan2d(int device, int W, int H, Plan2D *out)
 {
     if (W < 1 || H < 1) return IMRESH_ERR_INVALID;
+    if (!line_supported(W) || !line_supported(H)) return IMRESH_ERR_UNSUPPORTED_SIZE;
     std::lock_guard<std::recursive_mutex> lk(g_plan_mtx);
     auto r = g_lines.find({device, W}), c = g_lines.find({device, H});
     if (r == g_lines.end() || c == g_lines.end()) return IMRESH_ERR_NOT_INITIALIZED;
@@ -295,6 +297,7 @@ int row_grid(const Plan2D &p)
     case 4: L = row_rows_g4(n, v); break;
     case 5: L = row_rows_g5(n, v); break;
     case 6: L = row_rows_g6(n, v); break;
+    case 7: L = row_rows_g7(n, v); break;
     default: break;
     }
     return (p.H + L - 1) / L;
@@ -315,6 +318,7 @@ int launch_row(int mode, const Plan2D &p, RowArgs a, hipStream_t s)
     case 4: return launch_row_g4(mode, v, n, p.row, a, s);
     case 5: return launch_row_g5(mode, v, n, p.row, a, s);
     case 6: return launch_row_g6(mode, v, n, p.row, a, s);
+    case 7: return launch_row_g7(mode, v, n, p.row, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
     }
 }
@@ -375,6 +379,7 @@ int launch_col(int mode, const Plan2D &p, ColArgs a, hipStream_t s)
     case 4: return launch_col_g4(mode, v, n, p.col, a, s);
     case 5: return launch_col_g5(mode, v, n, p.col, a, s);
     case 6: return launch_col_g6(mode, v, n, p.col, a, s);
+    case 7: return launch_col_g7(mode, v, n, p.col, a, s);
     default: return IMRESH_ERR_UNSUPPORTED_SIZE;
     }
 }
@@ -390,6 +395,7 @@ int row_variants(const LinePlan &lp)
     case 4: return row_variants_g4(lp.n);
     case 5: return row_variants_g5(lp.n);
     case 6: return row_variants_g6(lp.n);
+    case 7: return row_variants_g7(lp.n);
     default: return 0;
     }
 }
@@ -405,6 +411,7 @@ int col_variants(const LinePlan &lp)
     case 4: return col_variants_g4(lp.n);
     case 5: return col_variants_g5(lp.n);
     case 6: return col_variants_g6(lp.n);
+    case 7: return col_variants_g7(lp.n);
     default: return 0;
     }
 }

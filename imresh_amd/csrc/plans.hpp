@@ -18,9 +18,10 @@
     X(5, 1280, 16, 16, 5) X(5, 1536, 16, 8, 4, 3) X(5, 2560, 16, 16, 5, 2)             \
     X(5, 3072, 16, 16, 4, 3)                                                           \
     X(6, 160, 16, 5, 2) X(6, 192, 16, 4, 3) X(6, 320, 16, 5, 4) X(6, 640, 16, 8, 5)   \
-    X(6, 5120, 16, 16, 5, 4) X(6, 6144, 16, 16, 8, 3)
+    X(6, 5120, 16, 16, 5, 4) X(6, 6144, 16, 16, 8, 3)                                 \
+    X(7, 16384, 16, 16, 8, 8)
 
-#define IMR_NUM_PLAN_GROUPS 7
+#define IMR_NUM_PLAN_GROUPS 8
 
 namespace imr {
 

@@ -339,7 +339,8 @@ int imresh_diffraction_intensity_device(float *io_dev, uint32_t width, uint32_t 
  * Asynchronous. */
 int imresh_poisson_modulus(float *modulus_dev, uint32_t width, uint32_t height, double photons,
                            uint64_t seed, void *stream);
-/* 1 if an FFT plan exists for this line length, else 0. */
+/* 1 if this line length has a transform (1 ... 8192: a compiled plan or
+ * Bluestein), else 0. */
 int imresh_fft_supported(uint32_t n);
 /* fftShift (src/imresh/libs/fftShift.cpp:42-61) in place on a device
  * nx x ny array of element_bytes = 4 (float) or 8 (complex float) elements:

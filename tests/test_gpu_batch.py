@@ -135,8 +135,8 @@ def test_batch_rejects_bad_input():
     imgs = (im.Image * 1)()
     imgs[0] = im.Image(None, 64, 64)
     assert L.imresh_shrinkwrap_batch(imgs, 1, None, None, None) == 1   # IMRESH_ERR_NULL
-    a = np.zeros((2, 4097), np.float32)
-    imgs[0] = im.Image(a.ctypes.data, 4097, 2)
+    a = np.zeros((2, 8193), np.float32)
+    imgs[0] = im.Image(a.ctypes.data, 8193, 2)
     assert L.imresh_shrinkwrap_batch(imgs, 1, None, None, None) == 3   # unsupported length
 
 

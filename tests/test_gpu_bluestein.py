@@ -20,9 +20,11 @@ pytestmark = pytest.mark.gpu
 B = im.DeviceBuffer
 L = im.lib
 
-# (h, w): primes, odd composites without a plan, both axes / one axis
+# (h, w): primes, odd composites without a plan, both axes / one axis;
+# lengths 4097-8191 use the 16384-point convolution
 SIZES = [(7, 7), (13, 31), (97, 97), (67, 131), (100, 256), (256, 100), (1000, 600),
-         (2039, 5), (3, 4093), (1500, 2048)]
+         (2039, 5), (3, 4093), (1500, 2048), (300, 5000), (5000, 300), (7, 8191), (8191, 3),
+         (4097, 64), (6000, 6000)]
 
 
 def _fft_gpu(x, direction):
@@ -63,13 +65,15 @@ def test_bluestein_closed_forms():
 
 
 def _state(w, h, index=0):
+    oracle.set_fft_threads(16)
     obj, M = syn.workload(w, h, n_shapes=6, central=min(w, h) // 2, index=index)
     g = oracle.fft2d(M.astype(np.complex64), +1, np.float32)
     mask = oracle.make_mask(g, 3.0, 0.04, np.float32)
     return obj, M, g, g.copy(), mask
 
 
-@pytest.mark.parametrize("w,h", [(97, 97), (131, 67), (600, 1000), (2048, 1500)])
+@pytest.mark.parametrize("w,h", [(97, 97), (131, 67), (600, 1000), (2048, 1500), (5000, 300), (8191, 7),
+                                 (13, 6007), (4500, 4500)])
 def test_bluestein_one_hio_iteration(w, h):
     obj, M, g, gp, mask = _state(w, h, index=w * 3 + h)
     dg, dgp = B.from_array(g), B.from_array(gp)
@@ -83,7 +87,7 @@ def test_bluestein_one_hio_iteration(w, h):
     assert e < 1e-5
 
 
-@pytest.mark.parametrize("w,h", [(97, 97), (100, 150)])
+@pytest.mark.parametrize("w,h", [(97, 97), (100, 150), (4100, 96)])
 def test_bluestein_short_reconstruction(w, h):
     obj, M = syn.workload(w, h, n_shapes=6, central=min(w, h) // 2, index=w + h)
     kw = dict(n_cycles=2, target_error=FLT_MIN, n_hio_cycles=2)

@@ -405,14 +405,18 @@ def test_fft2d_closed_forms():
 
 
 def test_fft_unsupported_size_is_an_error():
-    """Every length up to 4096 has a transform (own plan or Bluestein); the
-    compiled plans also cover 8192; anything else is an error."""
-    d = B(8 * 4097 * 2)
-    assert L().imresh_fft2d(d.ptr, d.ptr, 4097, 2, -1, None) == 3   # IMRESH_ERR_UNSUPPORTED_SIZE
-    assert L().imresh_fft2d(d.ptr, d.ptr, 2, 4097, -1, None) == 3
+    """Every length up to 8192 has a transform (own plan or Bluestein; the
+    16384-point convolution plan is not a line length itself); anything
+    else is an error."""
+    d = B(8 * 8193 * 2)
+    assert L().imresh_fft2d(d.ptr, d.ptr, 8193, 2, -1, None) == 3   # IMRESH_ERR_UNSUPPORTED_SIZE
+    assert L().imresh_fft2d(d.ptr, d.ptr, 2, 8193, -1, None) == 3
+    assert L().imresh_fft2d(d.ptr, d.ptr, 16384, 1, -1, None) == 3
     assert L().imresh_fft_supported(2048) == 1 and L().imresh_fft_supported(7) == 1
     assert L().imresh_fft_supported(4096) == 1 and L().imresh_fft_supported(8192) == 1
-    assert L().imresh_fft_supported(4097) == 0 and L().imresh_fft_supported(0) == 0
+    assert L().imresh_fft_supported(4097) == 1 and L().imresh_fft_supported(8191) == 1
+    assert L().imresh_fft_supported(8193) == 0 and L().imresh_fft_supported(0) == 0
+    assert L().imresh_fft_supported(16384) == 0
 
 
 @pytest.mark.parametrize("h,w", [(8, 8), (6, 10), (5, 3), (1, 7), (7, 1), (64, 96), (97, 131)])
@@ -464,7 +468,7 @@ def test_diffraction_intensity_closed_forms_and_errors():
     assert np.array_equal(im.diffraction_intensity(z), z)
     assert L().imresh_diffraction_intensity(None, w, h, None, 0) == 1           # IMRESH_ERR_NULL
     assert L().imresh_diffraction_intensity(d.ctypes.data, 0, h, None, 0) == 2  # IMRESH_ERR_INVALID
-    assert L().imresh_diffraction_intensity(d.ctypes.data, 4097, 1, None, 0) == 3
+    assert L().imresh_diffraction_intensity(d.ctypes.data, 8193, 1, None, 0) == 3
 
 
 def test_device_forward_model_feeds_reconstruction():

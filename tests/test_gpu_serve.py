@@ -88,7 +88,7 @@ def test_server_rejects_over_budget_and_bad_sizes():
         a = np.zeros((512, 512), np.float32)
         with pytest.raises(im.ImreshError):
             srv.submit(a)
-        b = np.zeros((2, 4097), np.float32)
+        b = np.zeros((2, 8193), np.float32)
         with pytest.raises(im.ImreshError):
             srv.submit(b)
 
