@@ -273,7 +273,7 @@ def _gen_batch_input(args):
                         index=20_000_000 + i)[1]
 
 
-def generate_inputs(batch_count: int, serve_count: int, light_count: int = 150, light_rate: float = 50.0):
+def generate_inputs(batch_count: int, serve_count: int, light_count: int = 1000, light_rate: float = 100.0):
     """All synthetic inputs, generated in worker processes BEFORE this
     process touches the GPU (workers are forked; a process that has
     initialized the GPU must not fork helpers)."""
@@ -403,13 +403,18 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, mode
     def run_server():
         with im.Server(hbm_budget_bytes=budget_gib << 30) as srv:
             ids = {}
+            depth = []
 
             def sub(k, io):
                 ids[srv.submit(io, params[k])] = k
+                if k % 10 == 0:
+                    st = srv.stats()
+                    depth.append(int(st.queued + st.active))
 
             _, ios = replay(sub, srv.drain)
             comp = srv.completions(count + 16)
             peak = srv.stats().peak_bytes
+            run_server.depth = depth
         lat = {ids[c[0]]: c[3] for c in comp}   # from submit (= arrival: submit never blocks)
         errs = sum(1 for c in comp if c[1] != 0)
         span = max(trace[k][0] + lat[k] / 1e3 for k in range(count))
@@ -445,12 +450,21 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, mode
     lat_s, span_s, ios_s, peak, bad = res_s[gs]
     lat_q, span_q, ios_q = res_q[gq]
     same = all(np.array_equal(a, b) for a, b in zip(ios_s, ios_q))
-    out = {"requests": count, "arrival_rate_per_s": rate, "sizes": "round(exp(U[ln128, ln2048]))",
+    half = count // 2
+    depth = getattr(run_server, "depth", [])
+    out = {"requests": count, "arrival_rate_per_s": rate if np.isfinite(rate) else "all submitted at t=0", "sizes": "round(exp(U[ln128, ln2048]))",
            "cycles": "5/10/20 x 20 HIO", "hbm_budget_gib": budget_gib, "peak_state_gib": round(peak / 2**30, 3),
            "server": {"recon_per_s": round(count / span_s, 2),
-                      "hio_iters_per_s": round(iters / span_s, 1), **_percentiles(lat_s), "errors": bad},
+                      "hio_iters_per_s": round(iters / span_s, 1), **_percentiles(lat_s), "errors": bad,
+                      "p50_ms_first_half": _percentiles(lat_s[:half])["p50_ms"],
+                      "p50_ms_second_half": _percentiles(lat_s[half:])["p50_ms"],
+                      "requests_in_system_at_arrivals": {"max": max(depth) if depth else None,
+                                                         "first_quarter_mean": round(float(np.mean(depth[:max(1, len(depth) // 4)])), 1) if depth else None,
+                                                         "last_quarter_mean": round(float(np.mean(depth[-max(1, len(depth) // 4):])), 1) if depth else None}},
            "task_queue": {"recon_per_s": round(count / span_q, 2),
-                          "hio_iters_per_s": round(iters / span_q, 1), **_percentiles(lat_q)},
+                          "hio_iters_per_s": round(iters / span_q, 1), **_percentiles(lat_q),
+                          "p50_ms_first_half": _percentiles(lat_q[:half])["p50_ms"],
+                          "p50_ms_second_half": _percentiles(lat_q[half:])["p50_ms"]},
            "server_cycle_graphs": bool(gs), "task_queue_cycle_graphs": bool(gq),
            "span_s_by_mode": {("graphs" if g else "eager"): {"server": round(res_s[g][1], 3),
                                                               "task_queue": round(res_q[g][1], 3)}
@@ -731,8 +745,11 @@ def main() -> int:
     serve = None
     if not args.no_serve:
         serve = serve_bench(L, trace, smods)
-        log("[bench] serving below capacity (50 requests/s)")
-        serve["below_capacity"] = serve_bench(L, light, lmods, rate=50.0, modes=(0,))
+        log("[bench] serving capacity: the same requests all at once")
+        serve["capacity"] = serve_bench(L, [(0.0, n, c) for _, n, c in trace], smods, rate=float("inf"),
+                                        modes=(0,))
+        log("[bench] serving steady state (1000 requests at 100/s)")
+        serve["steady_state"] = serve_bench(L, light, lmods, rate=100.0, modes=(0,))
 
     large = None
     if M8 is not None and not args.no_large:

@@ -2,6 +2,8 @@
 // library: status codes, launch descriptors and the line-FFT plan layout.
 #pragma once
 
+#include <cstdio>
+#include <cstdlib>
 #include <new>
 
 #include <hip/hip_ext.h>
@@ -20,8 +22,11 @@ constexpr int kMaxPasses = 8;
 #define IMR_HIP_CHECK(expr)                                              \
     do {                                                                 \
         hipError_t _e = (expr);                                          \
-        if (_e != hipSuccess) return imr::from_hip(_e);                  \
+        if (_e != hipSuccess) return imr::hip_fail(_e, #expr, __FILE__, __LINE__); \
     } while (0)
+
+// a HIP call's status as an imresh status (reported like IMR_HIP_CHECK)
+#define IMR_HIP_STATUS(expr) imr::hip_status((expr), #expr, __FILE__, __LINE__)
 
 #define IMR_CHECK(expr)                                                  \
     do {                                                                 \
@@ -36,6 +41,33 @@ inline int from_hip(hipError_t e)
     if (e == hipErrorOutOfMemory) return IMRESH_ERR_OUT_OF_MEMORY;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return IMRESH_ERR_NO_DEVICE;
     return IMRESH_ERR_HIP;
+}
+
+// a failed HIP call checked by IMR_HIP_CHECK: its status, and with
+// IMRESH_VERBOSE_ERRORS=1 in the environment a line on stderr naming the
+// call and the HIP error
+inline int hip_fail(hipError_t e, const char *expr, const char *file, int line)
+{
+    static const bool verbose = [] {
+        const char *v = std::getenv("IMRESH_VERBOSE_ERRORS");
+        return v != nullptr && v[0] == '1';
+    }();
+    if (verbose)
+        std::fprintf(stderr, "[imresh] %s:%d: %s -> %s (%s)\n", file, line, expr, hipGetErrorName(e),
+                     hipGetErrorString(e));
+    return from_hip(e);
+}
+
+inline int hip_status(hipError_t e, const char *expr, const char *file, int line)
+{
+    return e == hipSuccess ? IMRESH_OK : hip_fail(e, expr, file, line);
+}
+
+// status of the last kernel launch (hipGetLastError), reported as hip_fail
+inline int launch_status(const char *file, int line)
+{
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? IMRESH_OK : hip_fail(e, "kernel launch", file, line);
 }
 }  // namespace imr
 

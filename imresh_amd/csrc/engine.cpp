@@ -20,6 +20,8 @@
 // reconstruction into a no-op, so the host never waits inside the loop.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <thread>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -94,7 +96,6 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     }
     ws_layout(w, W, H, base);
     e = hipHostMalloc((void **) &w->host_stage, 4096, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&w->ready, hipEventDisableTiming);
     if (e != hipSuccess) {
         hipFree(w->base);
         if (w->host_stage) hipHostFree(w->host_stage);
@@ -112,9 +113,13 @@ void ws_release(Workspace *w)
     if (w->weights) hipFree(w->weights);
     if (w->host_stage) hipHostFree(w->host_stage);
     if (w->host_wpin) hipHostFree(w->host_wpin);
-    if (w->ready) hipEventDestroy(w->ready);
     if (w->gexec) hipGraphExecDestroy(w->gexec);
     delete w;
+}
+
+void ws_wait_idle(Workspace *w)
+{
+    while (w->busy.load(std::memory_order_acquire)) std::this_thread::sleep_for(std::chrono::microseconds(20));
 }
 
 size_t ws_bytes(int W, int H)
@@ -158,7 +163,7 @@ void trim_idle(size_t cap, std::vector<Workspace *> &out)
         for (auto &kv : g_ws_free)
             for (size_t i = 0; i < kv.second.size(); ++i) {
                 Workspace *w = kv.second[i];
-                if (w->pending && hipEventQuery(w->ready) != hipSuccess) continue;
+                if (w->busy.load(std::memory_order_acquire)) continue;
                 if (lru == nullptr || w->last_use < lru->last_use) {
                     lru = w;
                     list = &kv.second;
@@ -180,7 +185,7 @@ int ws_checkout(int device, int W, int H, hipStream_t s, Workspace **out)
         std::lock_guard<std::mutex> lk(g_ws_mtx);
         auto &v = g_ws_free[{device, W, H}];
         for (size_t i = 0; i < v.size() && w == nullptr; ++i)
-            if (!v[i]->pending || hipEventQuery(v[i]->ready) == hipSuccess) {
+            if (!v[i]->busy.load(std::memory_order_acquire)) {
                 w = v[i];
                 v.erase(v.begin() + (long) i);
             }
@@ -203,15 +208,30 @@ int ws_checkout(int device, int W, int H, hipStream_t s, Workspace **out)
         }
         return st;
     }
-    if (w->pending) IMR_HIP_CHECK(hipStreamWaitEvent(s, w->ready, 0));
+    // a busy workspace taken over (kMaxWorkspaces of this size in use): its
+    // previous user's work runs first
+    (void) s;
+    ws_wait_idle(w);
     *out = w;
     return IMRESH_OK;
 }
 
-void ws_return(Workspace *w, hipStream_t s)
+void mark_idle(void *p) { static_cast<Workspace *>(p)->busy.store(false, std::memory_order_release); }
+
+// synced: the caller has synchronized the stream after its last use of
+// the workspace (then it is idle at once; otherwise a host function behind
+// the queued work marks it idle)
+void ws_return(Workspace *w, hipStream_t s, bool synced = false)
 {
-    w->pending = hipEventRecord(w->ready, s) == hipSuccess;
-    if (!w->pending) hipStreamSynchronize(s);
+    if (synced) {
+        w->busy.store(false, std::memory_order_release);
+    } else {
+        w->busy.store(true, std::memory_order_relaxed);
+        if (hipLaunchHostFunc(s, mark_idle, w) != hipSuccess) {
+            hipStreamSynchronize(s);
+            w->busy.store(false, std::memory_order_release);
+        }
+    }
     std::vector<Workspace *> drop;
     {
         std::lock_guard<std::mutex> lk(g_ws_mtx);
@@ -296,7 +316,7 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
     std::vector<float> &host = w->host_weights;
     int R = 0;
     IMR_CHECK(padded_weights(q, host, &R));
-    if (w->pending && hipEventQuery(w->ready) != hipSuccess) IMR_HIP_CHECK(hipEventSynchronize(w->ready));
+    ws_wait_idle(w);   // a previous user's copy may still read the staging
     if (w->host_wpin_n < host.size()) {
         if (w->host_wpin) hipHostFree(w->host_wpin);
         w->host_wpin = nullptr;
@@ -362,7 +382,7 @@ int time_variant(bool row, int var, Plan2D p, Workspace *w, hipStream_t s, float
             st = row ? launch_row(ROW_MID, p, ra, s) : launch_col(col_mode, p, ca, s);
         }
         hipEventRecord(e1, s);
-        if (st == IMRESH_OK) st = from_hip(hipEventSynchronize(e1));
+        if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventSynchronize(e1));
         float t = 0.f;
         if (st == IMRESH_OK) hipEventElapsedTime(&t, e0, e1);
         best = std::min(best, t / reps);
@@ -521,10 +541,9 @@ int cycle_graph(Workspace *w, const Params &q, const Plan2D &p)
 {
     if (same_key(w, q, p)) return IMRESH_OK;
     if (w->gexec) {
-        // the previous user of the workspace may have replays of this graph
-        // still queued on its own stream (ws_checkout orders only the new
-        // stream behind them): wait for them before the graph is destroyed
-        if (w->pending) IMR_HIP_CHECK(hipEventSynchronize(w->ready));
+        // replays of this graph by the workspace's previous user must have
+        // run before it is destroyed (ws_checkout hands out idle workspaces)
+        ws_wait_idle(w);
         hipGraphExecDestroy(w->gexec);
         w->gexec = nullptr;
     }
@@ -568,7 +587,7 @@ int recon_cycle(Workspace *w, const float *M, const Params &q, unsigned c, hipSt
     IMR_CHECK(get_plan2d(w->device, w->W, w->H, &p));
     if (g_use_graphs) {
         IMR_CHECK(cycle_graph(w, q, p));
-        return from_hip(hipGraphLaunch(w->gexec, s));
+        return IMR_HIP_STATUS(hipGraphLaunch(w->gexec, s));
     }
     return enqueue_cycle(w, q, p, s);
 }
@@ -627,17 +646,100 @@ int read_result(Workspace *w, hipStream_t s, imresh_result *res)
 
 hipStream_t S(void *s) { return (hipStream_t) s; }
 
-// stream-ordered scratch for the per-operator reductions
+// Device scratch of the asynchronous per-operator entry points (the blur's
+// temporary plane, the Poisson simulation's sums), pooled per device: a
+// buffer is held by its call, then stays busy until a host function behind
+// the call's work on the caller's stream frees it.  (Stream-ordered
+// allocation would leave the allocator tracking the caller's stream, which
+// the caller may destroy.)
+struct PoolBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int dev = 0;
+    std::atomic<int> state{0};   // 0 free, 1 held by a call, 2 busy on a stream
+};
+std::mutex g_pb_mtx;
+std::vector<PoolBuf *> g_pb;
+
+void pool_buf_free(void *p) { static_cast<PoolBuf *>(p)->state.store(0, std::memory_order_release); }
+
 struct Scratch {
     void *p = nullptr;
+    PoolBuf *b = nullptr;
     hipStream_t s;
     explicit Scratch(hipStream_t st) : s(st) {}
-    int alloc(size_t bytes) { return from_hip(hipMallocAsync(&p, bytes, s)); }
+    int alloc(size_t bytes)
+    {
+        int dev;
+        IMR_CHECK(current_device(&dev));
+        {
+            std::lock_guard<std::mutex> lk(g_pb_mtx);
+            for (PoolBuf *x : g_pb) {
+                int fr = 0;
+                if (x->dev == dev && x->bytes >= bytes && x->state.compare_exchange_strong(fr, 1)) {
+                    b = x;
+                    break;
+                }
+            }
+        }
+        if (b == nullptr) {
+            size_t cap = 4096;
+            while (cap < bytes) cap <<= 1;
+            PoolBuf *x = new PoolBuf();
+            x->bytes = cap;
+            x->dev = dev;
+            x->state.store(1);
+            const hipError_t e = hipMalloc(&x->p, cap);
+            if (e != hipSuccess) {
+                delete x;
+                return IMR_HIP_STATUS(e);
+            }
+            std::lock_guard<std::mutex> lk(g_pb_mtx);
+            g_pb.push_back(x);
+            b = x;
+        }
+        p = b->p;
+        return IMRESH_OK;
+    }
     ~Scratch()
     {
-        if (p) hipFreeAsync(p, s);
+        if (b == nullptr) return;
+        b->state.store(2, std::memory_order_relaxed);
+        if (hipLaunchHostFunc(s, pool_buf_free, b) != hipSuccess) {
+            hipStreamSynchronize(s);
+            b->state.store(0, std::memory_order_release);
+        }
     }
 };
+
+// free pooled scratch buffers (imresh_release_cached) / their bytes
+void pool_bufs_release()
+{
+    std::vector<PoolBuf *> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_pb_mtx);
+        std::vector<PoolBuf *> keep;
+        for (PoolBuf *x : g_pb) {
+            int fr = 0;
+            if (x->state.compare_exchange_strong(fr, 1)) drop.push_back(x);
+            else keep.push_back(x);
+        }
+        g_pb.swap(keep);
+    }
+    for (PoolBuf *x : drop) {
+        hipFree(x->p);
+        delete x;
+    }
+}
+
+size_t pool_bufs_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_pb_mtx);
+    size_t t = 0;
+    for (PoolBuf *x : g_pb)
+        if (x->state.load() == 0) t += x->bytes;
+    return t;
+}
 
 int check_n(const void *p, uint32_t n)
 {
@@ -893,7 +995,7 @@ int tune_size(int device, int W, int H, hipStream_t s)
     Workspace *w;
     IMR_CHECK(ws_checkout(device, W, H, s, &w));
     const int st = autotune(device, W, H, s, w);
-    ws_return(w, s);
+    ws_return(w, s, st == IMRESH_OK);   // autotune synchronizes when it succeeds
     return st;
 }
 
@@ -953,32 +1055,32 @@ int imresh_device_count(int *count)
 int imresh_dev_alloc(void **dev, uint64_t bytes)
 {
     if (dev == nullptr) return IMRESH_ERR_NULL;
-    return from_hip(hipMalloc(dev, bytes ? bytes : 1));
+    return IMR_HIP_STATUS(hipMalloc(dev, bytes ? bytes : 1));
 }
 
-int imresh_dev_free(void *dev) { return from_hip(hipFree(dev)); }
+int imresh_dev_free(void *dev) { return IMR_HIP_STATUS(hipFree(dev)); }
 
 int imresh_memcpy_h2d(void *dev, const void *host, uint64_t bytes, void *stream)
 {
     if (!dev || !host) return IMRESH_ERR_NULL;
     IMR_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, S(stream)));
-    return from_hip(hipStreamSynchronize(S(stream)));
+    return IMR_HIP_STATUS(hipStreamSynchronize(S(stream)));
 }
 
 int imresh_memcpy_d2h(void *host, const void *dev, uint64_t bytes, void *stream)
 {
     if (!dev || !host) return IMRESH_ERR_NULL;
     IMR_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, S(stream)));
-    return from_hip(hipStreamSynchronize(S(stream)));
+    return IMR_HIP_STATUS(hipStreamSynchronize(S(stream)));
 }
 
 int imresh_memcpy_d2d(void *dst, const void *src, uint64_t bytes, void *stream)
 {
     if (!dst || !src) return IMRESH_ERR_NULL;
-    return from_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)));
+    return IMR_HIP_STATUS(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)));
 }
 
-int imresh_stream_sync(void *stream) { return from_hip(hipStreamSynchronize(S(stream))); }
+int imresh_stream_sync(void *stream) { return IMR_HIP_STATUS(hipStreamSynchronize(S(stream))); }
 
 int imresh_set_option(int option, int value)
 {
@@ -993,6 +1095,10 @@ int imresh_set_option(int option, int value)
             trim_idle(g_pool_cap, drop);
         }
         for (Workspace *x : drop) ws_release(x);
+        // the per-operator scratch and reduction slots are not kept under a
+        // new cap either
+        reduce_slots_release();
+        pool_bufs_release();
         return guarded([&] { return trim_idle_classes((size_t) value << 20); });
     }
     default: return IMRESH_ERR_INVALID;
@@ -1008,6 +1114,7 @@ int imresh_release_cached(void)
     }
     for (Workspace *x : drop) ws_release(x);
     reduce_slots_release();
+    pool_bufs_release();
     return guarded([] { return trim_idle_classes(0); });
 }
 
@@ -1020,7 +1127,7 @@ int imresh_cached_bytes(uint64_t *bytes)
         ws = g_idle_bytes;
     }
     // the pools' locks are never held together
-    *bytes = ws + idle_class_bytes() + reduce_slots_bytes();
+    *bytes = ws + idle_class_bytes() + reduce_slots_bytes() + pool_bufs_bytes();
     return IMRESH_OK;
 }
 
@@ -1036,7 +1143,7 @@ int imresh_stream_create(void **stream)
 int imresh_stream_destroy(void *stream)
 {
     if (stream == nullptr) return IMRESH_ERR_NULL;
-    return from_hip(hipStreamDestroy(S(stream)));
+    return IMR_HIP_STATUS(hipStreamDestroy(S(stream)));
 }
 
 static int shrinkwrap_device_impl(float *io_dev, uint32_t width, uint32_t height,
@@ -1056,7 +1163,7 @@ static int shrinkwrap_device_impl(float *io_dev, uint32_t width, uint32_t height
     int st = ws_tune(w, s);
     if (st == IMRESH_OK) st = run_reconstruction(w, io_dev, q, s);
     if (st == IMRESH_OK && result != nullptr) st = read_result(w, s, result);
-    ws_return(w, s);
+    ws_return(w, s, st == IMRESH_OK && result != nullptr);
     return st;
 }
 
@@ -1077,15 +1184,15 @@ static int shrinkwrap_host_impl(float *io_host, uint32_t width, uint32_t height,
     const size_t bytes = sizeof(float) * w->n;
     // tuning runs on the workspace's planes, so before the modulus lands in M
     int st = ws_tune(w, s);
-    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(w->M, io_host, bytes, hipMemcpyHostToDevice, s));
+    if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipMemcpyAsync(w->M, io_host, bytes, hipMemcpyHostToDevice, s));
     if (st == IMRESH_OK) st = run_reconstruction(w, w->M, q, s);
-    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(io_host, w->M, bytes, hipMemcpyDeviceToHost, s));
+    if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipMemcpyAsync(io_host, w->M, bytes, hipMemcpyDeviceToHost, s));
     if (st == IMRESH_OK) {
         imresh_result r;
         st = read_result(w, s, &r);
         if (st == IMRESH_OK && result) *result = r;
     }
-    ws_return(w, s);
+    ws_return(w, s, st == IMRESH_OK);
     return st;
 }
 
@@ -1125,21 +1232,21 @@ static int debug_cycle_mask_impl(const float *modulus_host, uint32_t width, uint
     const size_t n = w->n;
     int st = ws_tune(w, s);
     if (st == IMRESH_OK)
-        st = from_hip(hipMemcpyAsync(w->M, modulus_host, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        st = IMR_HIP_STATUS(hipMemcpyAsync(w->M, modulus_host, sizeof(float) * n, hipMemcpyHostToDevice, s));
     if (st == IMRESH_OK) st = recon_begin(w, w->M, q, s);
     for (unsigned c = 0; c < cycle && st == IMRESH_OK; ++c) st = recon_cycle(w, w->M, q, c, s);
     if (st == IMRESH_OK)
-        st = from_hip(hipMemcpyAsync(gprime_host, w->X, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
+        st = IMR_HIP_STATUS(hipMemcpyAsync(gprime_host, w->X, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
     if (st == IMRESH_OK) st = recon_cycle(w, w->M, q, cycle, s);
     if (st == IMRESH_OK)
-        st = from_hip(hipMemcpyAsync(mask_host, w->mask, n, hipMemcpyDeviceToHost, s));
+        st = IMR_HIP_STATUS(hipMemcpyAsync(mask_host, w->mask, n, hipMemcpyDeviceToHost, s));
     if (st == IMRESH_OK && blurred_host != nullptr)
-        st = from_hip(hipMemcpyAsync(blurred_host, w->B2, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        st = IMR_HIP_STATUS(hipMemcpyAsync(blurred_host, w->B2, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     if (st == IMRESH_OK) st = recon_stage_results(w, s);
-    if (st == IMRESH_OK) st = from_hip(hipStreamSynchronize(s));
+    if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipStreamSynchronize(s));
     if (st == IMRESH_OK && threshold_host != nullptr) *threshold_host = w->host_stage[0];
     if (st != IMRESH_OK) hipStreamSynchronize(s);
-    ws_return(w, s);
+    ws_return(w, s, true);
     return st;
 }
 
@@ -1263,8 +1370,8 @@ int hio_standalone_entry(float *io, const uint8_t *mask, uint32_t W, uint32_t H,
     const size_t n = w->n;
     const hipMemcpyKind in_kind = device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     int st = ws_tune(w, s);
-    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(w->M, io, sizeof(float) * n, in_kind, s));
-    if (st == IMRESH_OK) st = from_hip(hipMemcpyAsync(w->mask, mask, n, in_kind, s));
+    if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipMemcpyAsync(w->M, io, sizeof(float) * n, in_kind, s));
+    if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipMemcpyAsync(w->mask, mask, n, in_kind, s));
     if (st == IMRESH_OK) st = run_hio_standalone(w, n_cycles, target, beta, s, result);
     if (st == IMRESH_OK) {
         if (device_io) {
@@ -1272,11 +1379,11 @@ int hio_standalone_entry(float *io, const uint8_t *mask, uint32_t W, uint32_t H,
         } else {
             st = launch_copy_from_real(w->B2, w->X, n, s);
             if (st == IMRESH_OK)
-                st = from_hip(hipMemcpyAsync(io, w->B2, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+                st = IMR_HIP_STATUS(hipMemcpyAsync(io, w->B2, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         }
     }
-    if (st == IMRESH_OK) st = from_hip(hipStreamSynchronize(s));
-    ws_return(w, s);
+    if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipStreamSynchronize(s));
+    ws_return(w, s, st == IMRESH_OK);
     return st;
 }
 
@@ -1300,7 +1407,7 @@ int diffraction_entry(float *io, uint32_t W, uint32_t H, void *stream, bool devi
     const size_t n = w->n;
     float *plane = device_io ? io : w->M;
     int st = IMRESH_OK;
-    if (!device_io) st = from_hip(hipMemcpyAsync(plane, io, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    if (!device_io) st = IMR_HIP_STATUS(hipMemcpyAsync(plane, io, sizeof(float) * n, hipMemcpyHostToDevice, s));
     if (st == IMRESH_OK) {
         ColArgs ca{};
         ca.M = plane;
@@ -1316,9 +1423,9 @@ int diffraction_entry(float *io, uint32_t W, uint32_t H, void *stream, bool devi
         st = launch_row(ROW_FWD_NORM, p, ra, s);
     }
     if (st == IMRESH_OK && !device_io)
-        st = from_hip(hipMemcpyAsync(io, plane, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-    if (st == IMRESH_OK && !async) st = from_hip(hipStreamSynchronize(s));
-    ws_return(w, s);
+        st = IMR_HIP_STATUS(hipMemcpyAsync(io, plane, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (st == IMRESH_OK && !async) st = IMR_HIP_STATUS(hipStreamSynchronize(s));
+    ws_return(w, s, st == IMRESH_OK && !async);
     return st;
 }
 
@@ -1364,7 +1471,7 @@ int imresh_hio_iterations(float *gprime_dev, float *gprev_dev, const uint8_t *ma
             Workspace *w;
             IMR_CHECK(ws_checkout(dev, width, height, s, &w));
             const int st = autotune(dev, width, height, s, w);
-            ws_return(w, s);
+            ws_return(w, s, st == IMRESH_OK);   // autotune synchronizes when it succeeds
             IMR_CHECK(st);
         }
     }
@@ -1558,7 +1665,7 @@ int imresh_autotune(uint32_t width, uint32_t height, void *stream, int *row_vari
     Workspace *w;
     IMR_CHECK(ws_checkout(dev, width, height, s, &w));
     int st = autotune(dev, width, height, s, w);
-    ws_return(w, s);
+    ws_return(w, s, st == IMRESH_OK);   // autotune synchronizes when it succeeds
     IMR_CHECK(st);
     IMR_CHECK(get_plan2d(dev, width, height, &p));
     if (row_variant) *row_variant = p.rowVar;
@@ -1595,8 +1702,8 @@ int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms,
     // pass alternating on a filled state), then the cycle-boundary kernels
     auto timed = [&](int k, auto launch) {
         hipEvent_t a = nullptr, b = nullptr;
-        if (st == IMRESH_OK) st = from_hip(hipEventCreate(&a));
-        if (st == IMRESH_OK) st = from_hip(hipEventCreate(&b));
+        if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventCreate(&a));
+        if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventCreate(&b));
         if (st != IMRESH_OK) return;
         g_prof.start = a;
         g_prof.stop = b;
@@ -1618,7 +1725,7 @@ int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms,
             timed(2, [&] { return launch_blur(ba, s); });
             timed(3, [&] { return launch_row(ROW_LAST, p, ra, s); });
         }
-        if (st == IMRESH_OK) st = from_hip(hipStreamSynchronize(s));
+        if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipStreamSynchronize(s));
         for (size_t e = 0; e + 2 < ev.size(); e += 3) {
             const int k = (int) (intptr_t) ev[e + 2];
             float ms = 0.f;
@@ -1632,7 +1739,7 @@ int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms,
         ev.clear();
     }
     hipStreamSynchronize(s);
-    ws_return(w, s);
+    ws_return(w, s, true);
     if (st == IMRESH_OK)
         for (int k = 0; k < 4; ++k) out_ms[k] = cnt[k] ? (float) (acc[k] / cnt[k]) : 0.f;
     return st;
@@ -1662,15 +1769,15 @@ int imresh_profile_forward_model(uint32_t W, uint32_t H, uint32_t reps, float *o
         for (uint32_t i = 0; i < std::max(4u, reps) && st == IMRESH_OK; ++i) {
             for (int k = 0; k < 2 && st == IMRESH_OK; ++k) {
                 hipEvent_t a = nullptr, b = nullptr;
-                st = from_hip(hipEventCreate(&a));
-                if (st == IMRESH_OK) st = from_hip(hipEventCreate(&b));
+                st = IMR_HIP_STATUS(hipEventCreate(&a));
+                if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventCreate(&b));
                 if (st == IMRESH_OK) {
                     g_prof.start = a;
                     g_prof.stop = b;
                     st = k == 0 ? launch_col(COL_REAL_FWD, p, ca, s) : launch_row(ROW_FWD_NORM, p, ra, s);
                     g_prof = ProfEvents{};
                 }
-                if (st == IMRESH_OK) st = from_hip(hipEventSynchronize(b));
+                if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventSynchronize(b));
                 float ms = 0.f;
                 if (st == IMRESH_OK && round == 1 && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
                     acc[k] += ms;
@@ -1682,7 +1789,7 @@ int imresh_profile_forward_model(uint32_t W, uint32_t H, uint32_t reps, float *o
         }
     }
     hipStreamSynchronize(s);
-    ws_return(w, s);
+    ws_return(w, s, true);
     if (st == IMRESH_OK)
         for (int k = 0; k < 2; ++k) out_ms[k] = cnt[k] ? (float) (acc[k] / cnt[k]) : 0.f;
     return st;

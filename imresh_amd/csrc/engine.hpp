@@ -45,8 +45,12 @@ struct Workspace {
     float *host_wpin = nullptr;        // pinned staging of that table (the upload's source)
     size_t host_wpin_n = 0;            // its capacity in floats
     float *host_stage = nullptr;  // pinned, 64 floats + readback
-    hipEvent_t ready = nullptr;   // recorded when the last user's work is queued
-    bool pending = false;
+    // set when a user returns the workspace with work still queued on its
+    // stream, cleared by a host function enqueued behind that work.  (An
+    // event recorded on the user's stream would do, but HIP reads the
+    // stream an event was last recorded on when the event is queried or
+    // waited for, and the caller may have destroyed that stream by then.)
+    std::atomic<bool> busy{false};
     uint64_t last_use = 0;        // pool clock of the last return (idle trimming)
     // the captured cycle graph and the parameters it was captured with
     hipGraphExec_t gexec = nullptr;
@@ -60,6 +64,9 @@ struct Workspace {
 // returns the bytes the layout needs
 size_t ws_layout(Workspace *w, int W, int H, void *base);
 int ws_alloc(int device, int W, int H, Workspace **out);
+// blocks the calling thread until the workspace's previous user's work has
+// run (no-op for an idle workspace)
+void ws_wait_idle(Workspace *w);
 // batch class buffers: free idle ones whose work has finished, least
 // recently used first, until the idle bytes fit `cap`; bytes held idle
 int trim_idle_classes(size_t cap);

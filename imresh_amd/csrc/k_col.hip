@@ -147,7 +147,7 @@ int launch_variant(int mode, ColArgs a, hipStream_t s)
         break;
     default: return IMRESH_ERR_INVALID;
     }
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 // Column variants of a length (index: tile shape, then +count/2 for the

@@ -400,7 +400,7 @@ static int launch_blur_r(const BlurArgs &a, hipStream_t s)
     const dim3 grid((a.W + kBlurTX - 1) / kBlurTX, (a.H + kBlurTY - 1) / kBlurTY,
                     a.batch > 1 ? a.batch : 1);
     IMR_LAUNCH(k_blur2d<R>, grid, dim3(256), lds, s, a);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 template <int R>
@@ -456,14 +456,14 @@ int launch_blur_h_inplace(float *d, int W, int H, const float *w, int r, hipStre
         (const void *) k_blur_h_inplace, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
     if (attr != hipSuccess) return from_hip(attr);
     hipLaunchKernelGGL(k_blur_h_inplace, dim3(H), dim3(256), lds, s, d, W, w, r);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_max_finalize(const float *pmax, int np, float cut, float *thr, const int *done,
                         hipStream_t s)
 {
     hipLaunchKernelGGL(k_max_finalize, dim3(1), dim3(1024), 0, s, pmax, np, cut, thr, done);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float target, float *res,
@@ -471,7 +471,7 @@ int launch_err_finalize(const float *perr, const unsigned *pcnt, int np, float t
 {
     hipLaunchKernelGGL(k_err_finalize, dim3(1), dim3(1024), 0, s, perr, pcnt, np, target, res,
                        done, cycles);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 // batched error finalizer: workgroup b sums the np row-pass partials of
@@ -511,7 +511,7 @@ int launch_max_finalize_batch(const float *pmax, int np, float cutA, float cut, 
 {
     hipLaunchKernelGGL(k_max_finalize_batch, dim3(batch), dim3(1024), 0, s, pmax, np, cutA, cut, scal,
                        flags);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_err_finalize_batch(const float *perr, const unsigned *pcnt, int np, float target,
@@ -519,7 +519,7 @@ int launch_err_finalize_batch(const float *perr, const unsigned *pcnt, int np, f
 {
     hipLaunchKernelGGL(k_err_finalize_batch, dim3(batch), dim3(1024), 0, s, perr, pcnt, np, target,
                        scal, flags);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 // addRandomPhase (hybridInputOutput.cpp:186-195): X holds the forward FFT of
@@ -540,7 +540,7 @@ __global__ void __launch_bounds__(256) k_random_phase(float2 *X, const float *M,
 int launch_random_phase(float2 *X, const float *M, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_random_phase, dim3(grid_1d(n)), dim3(256), 0, s, X, M, n);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 // fftShift (fftShift.cpp:42-61): element (iy, ix), iy < ny/2, swaps with
@@ -570,7 +570,7 @@ int launch_fft_shift(void *d, unsigned nx, unsigned ny, unsigned elem_bytes, hip
         hipLaunchKernelGGL(k_fft_shift<float2>, dim3(grid_1d(half)), dim3(256), 0, s, (float2 *) d, nx, ny);
     else
         return IMRESH_ERR_INVALID;
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 // batch staging: image b of a pointer table <-> plane b of a batch buffer
@@ -598,51 +598,51 @@ int launch_gather_images(float *dst, const float *const *src, size_t n, int batc
 {
     const dim3 grid((unsigned) std::min<size_t>((n + 255) / 256, 1024), batch);
     hipLaunchKernelGGL(k_gather_images, grid, dim3(256), 0, s, dst, src, n);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_scatter_real(float *const *dst, const float2 *X, size_t n, int batch, hipStream_t s)
 {
     const dim3 grid((unsigned) std::min<size_t>((n + 255) / 256, 1024), batch);
     hipLaunchKernelGGL(k_scatter_real, grid, dim3(256), 0, s, dst, X, n);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_copy_from_real(float *out, const float2 *in, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy_from_real, dim3(grid_1d(n)), dim3(256), 0, s, out, in, n);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_copy_to_real(float2 *out, const float *in, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy_to_real, dim3(grid_1d(n)), dim3(256), 0, s, out, in, n);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_complex_norm(float *out, const float2 *in, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_complex_norm, dim3(grid_1d(n)), dim3(256), 0, s, out, in, n);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_apply_modulus(float2 *out, const float2 *in, const float *M, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_apply_modulus, dim3(grid_1d(n)), dim3(256), 0, s, out, in, M, n);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_hio_constraint(float2 *gp, const float2 *g, const float *mask, size_t n, float beta,
                           hipStream_t s)
 {
     hipLaunchKernelGGL(k_hio_constraint, dim3(grid_1d(n)), dim3(256), 0, s, gp, g, mask, n, beta);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_cutoff(float *d, size_t n, float t, float lo, float hi, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cutoff, dim3(grid_1d(n)), dim3(256), 0, s, d, n, t, lo, hi);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_reduce(int op, const float *d, size_t n, float *partials, int np, float *result,
@@ -664,7 +664,7 @@ int launch_reduce(int op, const float *d, size_t n, float *partials, int np, flo
         break;
     default: return IMRESH_ERR_INVALID;
     }
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_hio_error(const float2 *g, const void *mask, int mask_kind, size_t n, int invert,
@@ -681,7 +681,7 @@ int launch_hio_error(const float2 *g, const void *mask, int mask_kind, size_t n,
                            n, invert, perr, pcnt);
     hipLaunchKernelGGL(k_err_finalize, dim3(1), dim3(1024), 0, s, perr, pcnt, np, 0.f, res,
                        (int *) nullptr, (unsigned *) nullptr);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 // word w of the bit-packed mask from elements 32w .. 32w+31 of a byte mask
@@ -705,7 +705,7 @@ int launch_pack_mask(const uint8_t *mask, size_t n, uint32_t *bits, hipStream_t 
     const size_t nw = (n + 31) / 32;
     if (nw == 0) return IMRESH_OK;
     hipLaunchKernelGGL(k_pack_mask, dim3(grid_1d(nw)), dim3(256), 0, s, mask, n, bits);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 int launch_store_words(float *dst, const float *src, size_t n, hipStream_t s)
@@ -724,7 +724,7 @@ int launch_fill_state(float2 *X, float2 *G, uint8_t *mask, float *M, float *Bn, 
                       hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill_state, dim3(grid_1d(n)), dim3(256), 0, s, X, G, mask, M, Bn, n, W);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 }  // namespace imr

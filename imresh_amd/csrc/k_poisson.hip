@@ -71,7 +71,7 @@ int launch_poisson_modulus(float *m, size_t n, double photons, uint64_t seed, do
     const unsigned blocks = (unsigned) std::min<size_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_poisson_modulus, dim3(blocks), dim3(256), 0, s, m, n,
                        (const double *) (aux + kPoissonSumBlocks), photons, seed);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 }  // namespace imr

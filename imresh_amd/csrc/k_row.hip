@@ -200,7 +200,7 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
     case ROW_MID: IMR_LAUNCH((k_row<P, NL, RK_MID, PF, BL, STG>), grid, block, lds, s, a); break;
     default: return IMRESH_ERR_INVALID;
     }
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 // Row variants of a length (index: workgroup shape, then +count/2 for the

@@ -50,7 +50,7 @@ int launch_step8(int dir, const Step8Args &a, int batch, hipStream_t s)
     const dim3 grid((a.W + 63) / 64, a.H2 / 4, batch > 1 ? batch : 1), block(256);
     if (dir < 0) IMR_LAUNCH((k_step8<-1>), grid, block, 0, s, a);
     else IMR_LAUNCH((k_step8<+1>), grid, block, 0, s, a);
-    return from_hip(hipGetLastError());
+    return imr::launch_status(__FILE__, __LINE__);
 }
 
 }  // namespace imr

@@ -229,7 +229,7 @@ struct Server {
                     s2 = IMRESH_ERR_OUT_OF_MEMORY;
                 } else {
                     std::memcpy(r->pin, r->io, bytes);
-                    s2 = from_hip(hipMemcpyAsync(w->M, r->pin, bytes, hipMemcpyHostToDevice, r->lane));
+                    s2 = IMR_HIP_STATUS(hipMemcpyAsync(w->M, r->pin, bytes, hipMemcpyHostToDevice, r->lane));
                 }
             }
             if (s2 == IMRESH_OK) s2 = recon_begin(w, M, r->q, r->lane);
@@ -252,7 +252,7 @@ struct Server {
                 if (st == IMRESH_OK) st = recon_stage_results(r->w, r->lane);
                 hipEvent_t e = get_event();
                 if (st == IMRESH_OK && e == nullptr) st = IMRESH_ERR_HIP;
-                if (st == IMRESH_OK) st = from_hip(hipEventRecord(e, r->lane));
+                if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventRecord(e, r->lane));
                 if (st != IMRESH_OK) {
                     put_event(e);
                     r->status = st;
@@ -266,12 +266,12 @@ struct Server {
                 float *out = r->device_io ? r->io : r->w->M;
                 int st = recon_end(r->w, out, r->lane);
                 if (st == IMRESH_OK && !r->device_io)
-                    st = from_hip(hipMemcpyAsync(r->pin, r->w->M, sizeof(float) * r->w->n,
+                    st = IMR_HIP_STATUS(hipMemcpyAsync(r->pin, r->w->M, sizeof(float) * r->w->n,
                                                  hipMemcpyDeviceToHost, r->lane));
                 if (st == IMRESH_OK) st = recon_stage_results(r->w, r->lane);
                 r->fin = get_event();
                 if (st == IMRESH_OK && r->fin == nullptr) st = IMRESH_ERR_HIP;
-                if (st == IMRESH_OK) st = from_hip(hipEventRecord(r->fin, r->lane));
+                if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventRecord(r->fin, r->lane));
                 r->status = st;
                 r->finishing = true;
                 progress = true;

@@ -94,12 +94,14 @@ int imresh_stream_sync(void *stream);
  * replays each shrink-wrap cycle as a captured HIP graph, 0 launches its
  * kernels one by one; both compute the same bits.  IMRESH_OPT_POOL_MIB: the
  * device memory (MiB, default 32768) that idle per-size workspaces may keep
- * between calls; beyond it the least recently used idle ones are freed. */
+ * between calls; beyond it the least recently used idle ones are freed
+ * (setting it also frees the idle per-operator scratch). */
 enum { IMRESH_OPT_CYCLE_GRAPHS = 1, IMRESH_OPT_POOL_MIB = 2 };
 int imresh_set_option(int option, int value);
 /* Frees every idle pooled workspace and batch buffer whose work has
- * finished (all of them after the caller's streams are synchronized) and
- * the idle scratch slots of the synchronous reductions. */
+ * finished (all of them after the caller's streams are synchronized), the
+ * idle scratch slots of the synchronous reductions and the idle scratch
+ * buffers of the asynchronous per-operator entry points. */
 int imresh_release_cached(void);
 /* Device bytes held by idle pooled workspaces, batch buffers and reduction
  * scratch slots. */

@@ -52,3 +52,43 @@ def test_alternating_parameters_on_six_streams(graphs):
         im.set_option(im.OPT_CYCLE_GRAPHS, 1)
         for s in streams:
             im.stream_destroy(s)
+
+
+def test_pooled_state_outlives_destroyed_streams():
+    """Reconstructions on caller streams that are destroyed afterwards, with
+    servers (which create and destroy streams of their own) in between:
+    the pooled workspaces must not keep anything that refers to the
+    destroyed streams (HIP reads the stream an event was last recorded on
+    when the event is queried, so a pool that tracked its workspaces with
+    events recorded on caller streams failed here with
+    hipErrorCapturedEvent)."""
+    trace = syn.serve_trace(40, index=3)
+    mods = [syn.fast_modulus(n, i) for i, (_, n, _) in enumerate(trace)]
+    p = im.default_params(n_cycles=2, n_hio_cycles=5, target_error=FLT_MIN)
+    ref = []
+    for m in mods:
+        x = m.copy()
+        im.check(im.lib().imresh_shrinkwrap(x.ctypes.data, x.shape[1], x.shape[0], p, None, None))
+        ref.append(x)
+    im.set_option(im.OPT_CYCLE_GRAPHS, 0)
+    try:
+        for k in (4, 8, 16):
+            bufs = [im.DeviceBuffer.from_array(m) for m in mods]
+            with im.Server() as srv:
+                for b, m in zip(bufs, mods):
+                    srv.submit((b.ptr, m.shape[1], m.shape[0]), p)
+                srv.drain()
+            for b, m in zip(bufs, mods):
+                b.upload(m)
+            streams = [im.stream_create() for _ in range(k)]
+            for i, (b, m) in enumerate(zip(bufs, mods)):
+                im.check(im.lib().imresh_shrinkwrap_device(b.ptr, m.shape[1], m.shape[0], p, streams[i % k], None),
+                         f"{k} streams, request {i}")
+            for s in streams:
+                im.sync(s)
+            for s in streams:
+                im.stream_destroy(s)
+            for b, m, r in zip(bufs, mods, ref):
+                assert np.array_equal(b.download(np.float32, m.shape), r)
+    finally:
+        im.set_option(im.OPT_CYCLE_GRAPHS, 1)
