@@ -507,6 +507,44 @@ def large_bench(L, M8, steps: int = 2) -> dict:
             "iteration_alg_GBps_80B": round(80 * n * n * N_CYCLES * N_HIO / dt / 1e9, 1)}
 
 
+def medium_bench(L, M4, steps: int = 3) -> dict:
+    """4096 x 4096 (256 shapes in the central 2048^2, 1e10 photons), 20 x 20:
+    the iteration's working set (iterate, gPrev, modulus, mask: 21
+    B/element, 352 MB) exceeds the 256 MB Infinity Cache, so this rate is
+    the HBM-resident counterpart of the 2048^2 figure."""
+    n = M4.shape[0]
+    d_src = im.DeviceBuffer.from_array(M4)
+    d_io = im.DeviceBuffer(M4.nbytes)
+    p = im.default_params(n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
+
+    def step():
+        im.check(L.imresh_memcpy_d2d(d_io.ptr, d_src.ptr, M4.nbytes, None), "d2d")
+        im.check(L.imresh_shrinkwrap_device(d_io.ptr, n, n, p, None, None), "shrinkwrap 4096")
+
+    step()
+    im.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    im.sync()
+    dt = (time.perf_counter() - t0) / steps
+    rate = N_CYCLES * N_HIO / dt
+    ms = (im.ctypes.c_float * 4)()
+    im.check(L.imresh_profile_kernels(n, n, 8, ms, None), "profile 4096")
+    per = dict(zip(KERNEL_NAMES, [float(x) for x in ms]))
+    kernels = {k: {"ms": round(per[k], 4),
+                   "GB/s": round(KERNEL_BYTES[k] * n * n / (per[k] * 1e-3) / 1e9, 1),
+                   "frac": round(KERNEL_BYTES[k] * n * n / (per[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+               for k in KERNEL_NAMES}
+    return {"workload": "shrink-wrap 4096x4096 fp32, 20 cycles x 20 HIO, 256 shapes in central 2048^2, "
+                        "1e10 photons",
+            "hio_iters_per_s": round(rate, 1), "ms_per_reconstruction": round(dt * 1e3, 2),
+            "kernels": kernels,
+            "iteration_alg_GBps_77B": round(77 * n * n * rate / 1e9, 1),
+            "iteration_frac_77B": round(77 * n * n * rate / 1e9 / HBM_PEAK_GBS, 4),
+            "iteration_fused_GBps_53B": round(53 * n * n * rate / 1e9, 1)}
+
+
 def gpu_per_op(L, M: np.ndarray, reps: int = 20) -> dict:
     """The per-operator entry points (the reference's unfused operator
     sequence, one launch per operator) at the workload's size, ms per call
@@ -644,6 +682,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-count", type=int, default=300)
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-medium", action="store_true")
     ap.add_argument("--no-fft-compare", action="store_true")
     ap.add_argument("--no-forward-model", action="store_true")
     ap.add_argument("--no-small", action="store_true")
@@ -751,6 +790,11 @@ def main() -> int:
         log("[bench] serving steady state (1000 requests at 100/s)")
         serve["steady_state"] = serve_bench(L, light, lmods, rate=100.0, modes=(0,))
 
+    medium = None
+    if not args.no_medium:
+        log("[bench] 4096x4096 reconstruction (working set beyond the Infinity Cache)")
+        medium = medium_bench(L, syn.workload(4096, 4096, n_shapes=256, central=2048, photons=1e10)[1])
+
     large = None
     if M8 is not None and not args.no_large:
         log("[bench] 8192x8192 reconstruction")
@@ -799,7 +843,7 @@ def main() -> int:
                    "row_variant": rv.value, "col_variant": cv.value},
         "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "quality": quality, "small_256": small,
         "batched": batch,
-        "serving": serve, "large_8192": large, "gpu_per_op_ms": per_op, "fft_vs_rocfft": fftc,
+        "serving": serve, "medium_4096": medium, "large_8192": large, "gpu_per_op_ms": per_op, "fft_vs_rocfft": fftc,
         "forward_model": fwd,
     }
     print(json.dumps(line), flush=True)
