@@ -117,9 +117,15 @@ void ws_release(Workspace *w)
     delete w;
 }
 
-void ws_wait_idle(Workspace *w)
+bool ws_wait_idle(Workspace *w, double timeout_s)
 {
-    while (w->busy.load(std::memory_order_acquire)) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    const auto t0 = std::chrono::steady_clock::now();
+    while (w->busy.load(std::memory_order_acquire)) {
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > timeout_s)
+            return false;
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    return true;
 }
 
 size_t ws_bytes(int W, int H)
@@ -209,9 +215,25 @@ int ws_checkout(int device, int W, int H, hipStream_t s, Workspace **out)
         return st;
     }
     // a busy workspace taken over (kMaxWorkspaces of this size in use): its
-    // previous user's work runs first
+    // previous user's work runs first.  Should that work not finish in
+    // reasonable time (a caller stream destroyed with its work dropped would
+    // never run the host function), the workspace stays in the pool and a
+    // new one is allocated instead of waiting on.
     (void) s;
-    ws_wait_idle(w);
+    if (!ws_wait_idle(w, 10.0)) {
+        {
+            std::lock_guard<std::mutex> lk(g_ws_mtx);
+            g_ws_free[{device, W, H}].push_back(w);
+            g_idle_bytes += w->bytes;
+            ++g_ws_count[{device, W, H}];
+        }
+        const int st = ws_alloc(device, W, H, out);
+        if (st != IMRESH_OK) {
+            std::lock_guard<std::mutex> lk(g_ws_mtx);
+            --g_ws_count[{device, W, H}];
+        }
+        return st;
+    }
     *out = w;
     return IMRESH_OK;
 }
@@ -316,7 +338,9 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
     std::vector<float> &host = w->host_weights;
     int R = 0;
     IMR_CHECK(padded_weights(q, host, &R));
-    ws_wait_idle(w);   // a previous user's copy may still read the staging
+    // a previous user's copy may still read the staging (checked-out
+    // workspaces are idle; this only guards)
+    if (!ws_wait_idle(w, 10.0)) return IMRESH_ERR_INTERNAL;
     if (w->host_wpin_n < host.size()) {
         if (w->host_wpin) hipHostFree(w->host_wpin);
         w->host_wpin = nullptr;
@@ -543,7 +567,7 @@ int cycle_graph(Workspace *w, const Params &q, const Plan2D &p)
     if (w->gexec) {
         // replays of this graph by the workspace's previous user must have
         // run before it is destroyed (ws_checkout hands out idle workspaces)
-        ws_wait_idle(w);
+        if (!ws_wait_idle(w, 10.0)) return IMRESH_ERR_INTERNAL;
         hipGraphExecDestroy(w->gexec);
         w->gexec = nullptr;
     }

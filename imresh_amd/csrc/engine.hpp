@@ -65,8 +65,8 @@ struct Workspace {
 size_t ws_layout(Workspace *w, int W, int H, void *base);
 int ws_alloc(int device, int W, int H, Workspace **out);
 // blocks the calling thread until the workspace's previous user's work has
-// run (no-op for an idle workspace)
-void ws_wait_idle(Workspace *w);
+// run (no-op for an idle workspace); false after timeout_s seconds
+bool ws_wait_idle(Workspace *w, double timeout_s);
 // batch class buffers: free idle ones whose work has finished, least
 // recently used first, until the idle bytes fit `cap`; bytes held idle
 int trim_idle_classes(size_t cap);
