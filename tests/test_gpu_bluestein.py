@@ -113,3 +113,30 @@ def test_bluestein_batch_equals_single():
         x = M.copy()
         im.shrink_wrap(x, **kw)
         assert np.array_equal(io, x)
+
+
+def test_long_bluestein_batch_and_server_equal_single():
+    """Lengths 4097-8191 (16384-point convolution) in the batched and the
+    served path give the single reconstruction's bits, beside plan and
+    shorter Bluestein sizes."""
+    sizes = [(5000, 64), (64, 4500), (128, 128), (300, 97)]
+    Ms = [syn.workload(w, h, n_shapes=4, central=min(w, h) // 2, index=50 + i)[1]
+          for i, (w, h) in enumerate(sizes)]
+    kw = dict(n_cycles=2, target_error=FLT_MIN, n_hio_cycles=3)
+    alone = []
+    for M in Ms:
+        x = M.copy()
+        im.shrink_wrap(x, **kw)
+        alone.append(x)
+    ios = [M.copy() for M in Ms]
+    im.shrink_wrap_batch(ios, im.default_params(**kw))
+    for a, b, s in zip(alone, ios, sizes):
+        assert np.array_equal(a, b), s
+    ios = [M.copy() for M in Ms]
+    with im.Server() as srv:
+        for io in ios:
+            srv.submit(io, im.default_params(**kw))
+        srv.drain()
+        assert all(c[1] == 0 for c in srv.completions())
+    for a, b, s in zip(alone, ios, sizes):
+        assert np.array_equal(a, b), s
