@@ -109,9 +109,9 @@ void dft_any(std::vector<std::complex<double>> &a)
 }
 
 // convolution length of a Bluestein line: the smallest power of two
-// >= max(64, 2n-1).  (Smooth lengths 3*2^k / 5*2^k pad less but measured
-// slower per line on gfx950: their plans need thread counts that are not
-// multiples of the wave size.)
+// >= max(64, 2n-1).  (Smooth lengths 3*2^k pad less but measured slower per
+// image on gfx950, with their plans ending in a radix-12 or radix-6 pass as
+// well as with radix-3 passes.)
 int blue_m(int n)
 {
     int m = 64;
@@ -181,6 +181,11 @@ int get_line_plan(int device, int n, LinePlan *out)
         lp.blue = 1;
         lp.m = m;
         lp.bs = dev;
+        // the four-step column pass belongs to columns of the convolution
+        // length itself: a Bluestein column of length n < m keeps the
+        // one-pass kernels (the four-step outer steps would address m rows)
+        lp.fs_h2 = 0;
+        lp.fs_tw = nullptr;
         g_lines[{device, n}] = lp;
         *out = lp;
         return IMRESH_OK;

@@ -73,7 +73,7 @@ def _state(w, h, index=0):
 
 
 @pytest.mark.parametrize("w,h", [(97, 97), (131, 67), (600, 1000), (2048, 1500), (5000, 300), (8191, 7),
-                                 (13, 6007), (4500, 4500)])
+                                 (13, 6007), (4500, 4500), (97, 3000), (3000, 2049), (5, 4095)])
 def test_bluestein_one_hio_iteration(w, h):
     obj, M, g, gp, mask = _state(w, h, index=w * 3 + h)
     dg, dgp = B.from_array(g), B.from_array(gp)
@@ -87,7 +87,7 @@ def test_bluestein_one_hio_iteration(w, h):
     assert e < 1e-5
 
 
-@pytest.mark.parametrize("w,h", [(97, 97), (100, 150), (4100, 96)])
+@pytest.mark.parametrize("w,h", [(97, 97), (100, 150), (4100, 96), (3000, 96), (96, 3000)])
 def test_bluestein_short_reconstruction(w, h):
     obj, M = syn.workload(w, h, n_shapes=6, central=min(w, h) // 2, index=w + h)
     kw = dict(n_cycles=2, target_error=FLT_MIN, n_hio_cycles=2)
