@@ -172,6 +172,47 @@ __device__ __forceinline__ void bf_odd(cf *v)
 }
 
 template <int R, int DIR>
+__device__ __forceinline__ void bfly(cf *v);
+
+// inverse of a modulo m (a, m coprime, m small), for the index maps below
+__host__ __device__ constexpr int inv_mod(int a, int m)
+{
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return m == 1 ? 0 : -1;
+}
+
+// Radix N1*N2 with N1, N2 coprime (6, 10, 12, 15) by the prime-factor
+// (Good-Thomas) algorithm: with input n = (N2 n1 + N1 n2) mod N and output
+// k = (N2 (N2^-1 mod N1) k1 + N1 (N1^-1 mod N2) k2) mod N, the length-N DFT
+// is N2 length-N1 DFTs over n1 followed by N1 length-N2 DFTs over n2, with
+// no twiddle factors between them.  The index maps are compile-time
+// register permutations.
+template <int N1, int N2, int DIR>
+__device__ __forceinline__ void bf_pfa(cf *v)
+{
+    constexpr int N = N1 * N2;
+    constexpr int i1 = inv_mod(N2 % N1, N1), i2 = inv_mod(N1 % N2, N2);
+    static_assert(i1 > 0 && i2 > 0, "prime-factor radices must be coprime");
+    cf a[N1][N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+        cf c[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) c[n1] = v[(N2 * n1 + N1 * n2) % N];
+        bfly<N1, DIR>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) a[k1][n2] = c[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+        bfly<N2, DIR>(a[k1]);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) v[(N2 * i1 * k1 + N1 * i2 * k2) % N] = a[k1][k2];
+    }
+}
+
+template <int R, int DIR>
 __device__ __forceinline__ void bfly(cf *v)
 {
     if constexpr (R == 1) (void) v;   // length-1 line: identity
@@ -179,6 +220,10 @@ __device__ __forceinline__ void bfly(cf *v)
     else if constexpr (R == 4) bf4<DIR>(v);
     else if constexpr (R == 8) bf8<DIR>(v);
     else if constexpr (R == 16) bf16<DIR>(v);
+    else if constexpr (R == 6) bf_pfa<2, 3, DIR>(v);
+    else if constexpr (R == 10) bf_pfa<2, 5, DIR>(v);
+    else if constexpr (R == 12) bf_pfa<4, 3, DIR>(v);
+    else if constexpr (R == 15) bf_pfa<3, 5, DIR>(v);
     else bf_odd<R, DIR>(v);
 }
 

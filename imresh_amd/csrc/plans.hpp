@@ -8,17 +8,17 @@
 
 #include "common.hpp"
 
-#define IMR_PLANS(X)                                                           \
+#define IMR_PLANS(X)                                                            \
     X(0, 1, 1) X(0, 2, 2) X(0, 3, 3) X(0, 4, 4) X(0, 5, 5) X(0, 8, 8)           \
-    X(0, 12, 4, 3) X(0, 16, 16) X(0, 32, 8, 4) X(0, 60, 4, 5, 3) X(0, 64, 8, 8) \
+    X(0, 12, 12) X(0, 16, 16) X(0, 32, 8, 4) X(0, 60, 4, 15) X(0, 64, 8, 8)     \
     X(1, 128, 16, 8) X(1, 256, 16, 16) X(1, 512, 8, 8, 8)                       \
     X(2, 1024, 16, 8, 8) X(2, 2048, 16, 16, 8)                                  \
     X(3, 4096, 16, 16, 16) X(3, 8192, 16, 8, 8, 8)                              \
-    X(4, 96, 8, 4, 3) X(4, 200, 8, 5, 5) X(4, 384, 16, 8, 3) X(4, 768, 16, 16, 3)     \
-    X(5, 1280, 16, 16, 5) X(5, 1536, 16, 8, 4, 3) X(5, 2560, 16, 16, 5, 2)             \
-    X(5, 3072, 16, 16, 4, 3)                                                           \
-    X(6, 160, 16, 5, 2) X(6, 192, 16, 4, 3) X(6, 320, 16, 5, 4) X(6, 640, 16, 8, 5)   \
-    X(6, 5120, 16, 16, 5, 4) X(6, 6144, 16, 16, 8, 3)                                 \
+    X(4, 96, 8, 12) X(4, 200, 8, 5, 5) X(4, 384, 16, 8, 3) X(4, 768, 16, 16, 3) \
+    X(5, 1280, 16, 16, 5) X(5, 1536, 16, 16, 6) X(5, 2560, 16, 16, 10)          \
+    X(5, 3072, 16, 16, 12)                                                      \
+    X(6, 160, 16, 10) X(6, 192, 16, 12) X(6, 320, 16, 5, 4) X(6, 640, 16, 8, 5) \
+    X(6, 5120, 16, 16, 5, 4) X(6, 6144, 16, 16, 8, 3)                           \
     X(7, 16384, 16, 16, 8, 8)
 
 #define IMR_NUM_PLAN_GROUPS 8
