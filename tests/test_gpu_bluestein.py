@@ -116,10 +116,12 @@ def test_bluestein_batch_equals_single():
 
 
 def test_long_bluestein_batch_and_server_equal_single():
-    """Lengths 4097-8191 (16384-point convolution) in the batched and the
-    served path give the single reconstruction's bits, beside plan and
-    shorter Bluestein sizes."""
-    sizes = [(5000, 64), (64, 4500), (128, 128), (300, 97)]
+    """Lengths 2049-8191 (8192- and 16384-point convolutions) and 8192-row
+    images (four-step column pass, two images of one size in one batch
+    class) in the batched and the served path give the single
+    reconstruction's bits, beside plan and shorter Bluestein sizes."""
+    sizes = [(5000, 64), (64, 4500), (128, 128), (300, 97), (16, 8192), (16, 8192), (40, 3000),
+             (3000, 40)]
     Ms = [syn.workload(w, h, n_shapes=4, central=min(w, h) // 2, index=50 + i)[1]
           for i, (w, h) in enumerate(sizes)]
     kw = dict(n_cycles=2, target_error=FLT_MIN, n_hio_cycles=3)

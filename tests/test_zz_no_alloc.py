@@ -57,6 +57,17 @@ def test_noalloc_equals_device_entry_two_calls():
           f"300x200: {s2.cycles_run} cycles, error {s2.last_error:.6g}")
 
 
+def test_noalloc_four_step_and_long_bluestein():
+    """8192-row images (four-step column pass) and 3000-long lines
+    (8192-point Bluestein convolutions) through the fixed array."""
+    p = im.default_params(n_cycles=2, n_hio_cycles=3, target_error=FLT_MIN)
+    for i, (w, h) in enumerate([(16, 8192), (40, 3000), (3000, 40)]):
+        _, M = syn.workload(w, h, n_shapes=4, central=min(w, h) // 2, index=45 + i)
+        a, ra = _regular(M, p)
+        b, rb = _noalloc(M, p)
+        _same(a, ra, b, rb)
+
+
 def test_noalloc_full_config_2048_and_early_exit():
     """The 2048^2 workload (20 x 20) fits the array; an early exit as well."""
     _, M = syn.config_2048(0)
