@@ -39,7 +39,8 @@ def test_first_guess_matches_oracle(w, h):
     assert r.last_masked == st.last_masked
 
 
-@pytest.mark.parametrize("w,h,n", [(64, 64, 1), (128, 96, 3), (256, 256, 2)])
+@pytest.mark.parametrize("w,h,n", [(64, 64, 1), (128, 96, 3), (256, 256, 2), (97, 131, 2), (16, 8192, 2),
+                                   (3000, 40, 2), (40, 3000, 2)])
 def test_short_runs_match_oracle(w, h, n):
     obj, M, mask = _case(w, h, 7 * w + h)
     x = M.copy()

@@ -342,7 +342,7 @@ def test_blur_too_wide_kernel_is_an_error():
 FFT_SIZES = [(1, 1), (2, 4), (8, 8), (3, 5), (12, 60), (16, 32), (64, 64), (128, 256),
              (96, 200), (384, 768), (512, 512), (1024, 64), (2048, 16), (16, 2048),
              (4096, 8), (8, 8192), (160, 192), (320, 640), (1280, 1536), (2560, 8), (8, 3072),
-             (5120, 4), (4, 6144)]
+             (5120, 4), (4, 6144), (8192, 8), (3000, 40)]
 
 
 def _fft_gpu(x, direction):
@@ -433,7 +433,7 @@ def test_fft_shift_matches_oracle(h, w):
 
 # ---------------------------------------------------------------- forward model
 DIFF_SIZES = [(64, 64), (256, 256), (2048, 2048), (64, 96), (97, 131), (300, 200),
-              (1280, 1536), (8, 8192), (1, 7)]
+              (1280, 1536), (8, 8192), (1, 7), (8192, 16), (3000, 40)]
 
 
 @pytest.mark.parametrize("h,w", DIFF_SIZES)
