@@ -122,10 +122,10 @@ def parity_check(L, M: np.ndarray) -> dict:
         oracle's mask update of the same g': elements equal (bit-exact bar);
       * one fused HIO iteration from that state against the float64 oracle:
         relative L2 (bar 1e-5), with the float32 oracle's own distance;
-      * a full 20 x 20 reconstruction of a converging 256^2 workload against
-        the float64 oracle's end result (tests/golden/converging.npz): NCC to
-        the object (bar 1e-3), the object's relative L2 (bar 1e-3) and the
-        HIO error."""
+      * full 20 x 20 reconstructions of converging 256^2 and 300^2 workloads
+        (300 has no compiled plan: Bluestein transforms) against the float64
+        oracle's end results (tests/golden/converging.npz): NCC to the object
+        (bar 1e-3), the object's relative L2 (bar 1e-3) and the HIO error."""
     import oracle  # CPU checker; only the baseline legs use it
 
     oracle.set_fft_threads(min(16, os.cpu_count() or 8))
@@ -152,17 +152,21 @@ def parity_check(L, M: np.ndarray) -> dict:
     out["one_iteration_rel_l2_vs_fp64"] = float(f"{syn.rel_l2(g_gpu, g64):.3e}")
     out["one_iteration_rel_l2_fp32_oracle_vs_fp64"] = float(f"{syn.rel_l2(g32, g64):.3e}")
     z = np.load(os.path.join(ROOT, "tests", "golden", "converging.npz"), allow_pickle=False)
-    n, shapes, central, idx = (int(v) for v in z["c256a_spec"])
-    obj, Mc = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
-    x = Mc.copy()
-    r = im.shrink_wrap(x, n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
-    y0, x0, h, w = (int(v) for v in z["c256a_box"])
-    ref = z["c256a_crop64"].astype(np.float64)
-    out["converging_256"] = {
-        "ncc_gpu": round(syn.ncc(obj, x), 9), "ncc_fp64": round(float(z["c256a_ncc64"]), 9),
-        "object_rel_l2_vs_fp64": float(f"{np.linalg.norm(x[y0:y0 + h, x0:x0 + w] - ref) / np.linalg.norm(ref):.3e}"),
-        "hio_error_gpu": float(r.last_error), "hio_error_fp64": float(z["c256a_err64"]),
-        "hio_error_fp32_oracle": float(z["c256a_err32"]), "hio_error_first_cycle_fp64": float(z["c256a_err64_c1"])}
+    for name, key in (("c256a", "converging_256"), ("b300", "converging_300_bluestein")):
+        n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
+        obj, Mc = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
+        x = Mc.copy()
+        r = im.shrink_wrap(x, n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
+        y0, x0, h, w = (int(v) for v in z[f"{name}_box"])
+        st = int(z[f"{name}_step"])
+        ref = z[f"{name}_crop64"].astype(np.float64)
+        out[key] = {
+            "ncc_gpu": round(syn.ncc(obj, x), 9), "ncc_fp64": round(float(z[f"{name}_ncc64"]), 9),
+            "object_rel_l2_vs_fp64":
+                float(f"{np.linalg.norm(x[y0:y0 + h:st, x0:x0 + w:st] - ref) / np.linalg.norm(ref):.3e}"),
+            "hio_error_gpu": float(r.last_error), "hio_error_fp64": float(z[f"{name}_err64"]),
+            "hio_error_fp32_oracle": float(z[f"{name}_err32"]),
+            "hio_error_first_cycle_fp64": float(z[f"{name}_err64_c1"])}
     return out
 
 

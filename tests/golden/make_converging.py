@@ -16,7 +16,9 @@ Per workload <w> (plain arrays, allow_pickle=False):
   <w>_masked    masked count of the last cycle (float64)
   <w>_ncc64, <w>_ncc32   NCC of the end result to the known object
   <w>_box       [y0, x0, h, w] of the region holding the reconstruction
-  <w>_crop64    the float64 end result inside the box (stored as float32)
+  <w>_step      pixel stride of the stored crop (1: every pixel)
+  <w>_crop64    the float64 end result inside the box, every step-th row
+                and column (stored as float32)
   <w>_norm64    L2 norm of the whole float64 end result
   <w>_out64     L2 norm of the float64 end result outside the box
 Runs for ~10 minutes (the 2048^2 runs take most of it).
@@ -34,13 +36,19 @@ import oracle  # noqa: E402
 from imresh_amd import synthetic as syn  # noqa: E402
 
 FLT_MIN = float(np.finfo(np.float32).tiny)
-# name -> (width, n_shapes, central, index): found by running the oracle on
-# seeds of the benchmark's object generator with 1-2 shapes (objects with
-# more shapes stagnate, differently in every precision)
+# name -> (width, n_shapes, central, index, step): found by running the
+# oracle on seeds of the benchmark's object generator with 1-2 shapes
+# (objects with more shapes stagnate, differently in every precision); b300
+# and b1000 are lengths without a compiled plan (Bluestein transforms of 1024
+# and 2048 points on the GPU).  b1000's reconstruction spans the whole image
+# (the object lands across its edges), so its crop keeps every 4th row and
+# column.
 WORKLOADS = {
-    "c256a": (256, 1, 128, 100),
-    "c256b": (256, 2, 64, 101),
-    "c2048": (2048, 2, 512, 101),
+    "c256a": (256, 1, 128, 100, 1),
+    "c256b": (256, 2, 64, 101, 1),
+    "c2048": (2048, 2, 512, 101, 1),
+    "b300": (300, 1, 150, 102, 1),
+    "b1000": (1000, 1, 500, 101, 4),
 }
 
 
@@ -58,7 +66,7 @@ def box_of(x, margin=4):
 def main():
     oracle.set_fft_threads(int(os.environ.get("OMP_NUM_THREADS", "8")))
     out = {}
-    for name, (n, shapes, central, idx) in WORKLOADS.items():
+    for name, (n, shapes, central, idx, step) in WORKLOADS.items():
         obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
         kw = dict(n_cycles=20, target_error=FLT_MIN, n_hio_cycles=20)
         x64, s64 = oracle.shrink_wrap(M, dtype=np.float64, **kw)
@@ -74,7 +82,8 @@ def main():
         out[f"{name}_ncc64"] = np.float64(syn.ncc(obj, x64))
         out[f"{name}_ncc32"] = np.float64(syn.ncc(obj, x32))
         out[f"{name}_box"] = b
-        out[f"{name}_crop64"] = x64[y0:y0 + h, x0:x0 + w].astype(np.float32)
+        out[f"{name}_step"] = np.int64(step)
+        out[f"{name}_crop64"] = x64[y0:y0 + h:step, x0:x0 + w:step].astype(np.float32)
         out[f"{name}_norm64"] = np.float64(np.linalg.norm(x64))
         inside = np.zeros(x64.shape, bool)
         inside[y0:y0 + h, x0:x0 + w] = True

@@ -2,17 +2,18 @@
 (GPU).
 
 tests/golden/converging.npz (make_converging.py) holds the float64 oracle's
-end results of three workloads of the benchmark's object generator (256^2
-with one and two shapes, 2048^2 with two shapes, no Poisson noise) on which
-shrink-wrap converges to the known object in every precision, so the end
-result no longer depends on rounding.  The GPU reconstruction of the same
+end results of five workloads of the benchmark's object generator (256^2
+with one and two shapes, 2048^2 with two shapes, and 300^2 and 1000^2 with
+one shape -- lengths without a compiled plan, Bluestein transforms on the
+GPU; no Poisson noise) on which shrink-wrap converges to the known object in
+every precision, so the end result no longer depends on rounding.  The GPU reconstruction of the same
 modulus must match it:
 
 * NCC to the known object within 1e-3 of the float64 oracle's (the
   benchmark definition's tolerance; both are 1.000000 here);
 * the reconstruction itself within 1e-3 relative L2 of the float64 end
-  result (inside the object's box; outside it the residual stays as small
-  as the oracle's);
+  result (inside the object's box, on the stored stride of its pixels;
+  outside it the residual stays as small as the oracle's);
 * the last cycle's masked count equal to the float64 oracle's;
 * the final HIO error sqrtf(sum_out |g'|) / n_out: at convergence it
   measures a residual outside the support of ~1e-5 of the reconstruction's
@@ -34,7 +35,7 @@ from imresh_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 FIX = os.path.join(os.path.dirname(__file__), "golden", "converging.npz")
-NAMES = ["c256a", "c256b", "c2048"]
+NAMES = ["c256a", "c256b", "c2048", "b300", "b1000"]
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -49,7 +50,8 @@ def test_full_reconstruction_matches_float64_oracle(name):
     n64 = float(z[f"{name}_ncc64"])
     ncc = syn.ncc(obj, x)
     y0, x0, h, w = (int(v) for v in z[f"{name}_box"])
-    crop = x[y0:y0 + h, x0:x0 + w].astype(np.float64)
+    st = int(z[f"{name}_step"])
+    crop = x[y0:y0 + h:st, x0:x0 + w:st].astype(np.float64)
     ref = z[f"{name}_crop64"].astype(np.float64)
     rel = np.linalg.norm(crop - ref) / np.linalg.norm(ref)
     inside = np.zeros(x.shape, bool)

@@ -199,25 +199,28 @@ def test_hio_iteration_outside_support_closed_form():
     assert np.max(np.abs(g1 - g.size * exp_gp)) <= 1e-12 * g.size
 
 
-def test_converging_fixture_float32_within_the_gpu_bounds():
+@pytest.mark.parametrize("name", ["c256a", "b300"])
+def test_converging_fixture_float32_within_the_gpu_bounds(name):
     """tests/golden/converging.npz (make_converging.py): the float64 oracle
-    reproduces its 256^2 end result, and the float32 oracle -- the
-    reference's own precision -- meets the bounds the GPU test applies
-    (NCC within 1e-3, the reconstruction within 1e-3 relative L2)."""
+    reproduces its end result (256^2, and 300^2 whose length has no compiled
+    GPU plan), and the float32 oracle -- the reference's own precision --
+    meets the bounds the GPU test applies (NCC within 1e-3, the
+    reconstruction within 1e-3 relative L2)."""
     import os
     from imresh_amd import synthetic as syn
     z = np.load(os.path.join(os.path.dirname(__file__), "golden", "converging.npz"), allow_pickle=False)
     oracle.set_fft_threads(8)
-    n, shapes, central, idx = (int(v) for v in z["c256a_spec"])
+    n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
     obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
     kw = dict(n_cycles=20, target_error=np.finfo(np.float32).tiny, n_hio_cycles=20)
     x64, s64 = oracle.shrink_wrap(M, dtype=np.float64, **kw)
-    y0, x0, h, w = (int(v) for v in z["c256a_box"])
-    assert np.array_equal(x64[y0:y0 + h, x0:x0 + w].astype(np.float32), z["c256a_crop64"])
-    assert s64.last_error == np.float32(z["c256a_err64"])
+    y0, x0, h, w = (int(v) for v in z[f"{name}_box"])
+    st = int(z[f"{name}_step"])
+    assert np.array_equal(x64[y0:y0 + h:st, x0:x0 + w:st].astype(np.float32), z[f"{name}_crop64"])
+    assert s64.last_error == np.float32(z[f"{name}_err64"])
     x32, s32 = oracle.shrink_wrap(M, dtype=np.float32, **kw)
-    ref = z["c256a_crop64"].astype(np.float64)
-    rel = np.linalg.norm(x32[y0:y0 + h, x0:x0 + w] - ref) / np.linalg.norm(ref)
+    ref = z[f"{name}_crop64"].astype(np.float64)
+    rel = np.linalg.norm(x32[y0:y0 + h:st, x0:x0 + w:st] - ref) / np.linalg.norm(ref)
     assert rel <= 1e-3
-    assert abs(syn.ncc(obj, x32) - float(z["c256a_ncc64"])) <= 1e-3
-    assert abs(s32.last_error - float(z["c256a_err64"])) <= 1e-3 * float(z["c256a_err64_c1"])
+    assert abs(syn.ncc(obj, x32) - float(z[f"{name}_ncc64"])) <= 1e-3
+    assert abs(s32.last_error - float(z[f"{name}_err64"])) <= 1e-3 * float(z[f"{name}_err64_c1"])
