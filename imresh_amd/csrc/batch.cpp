@@ -86,10 +86,10 @@ void trim_classes(size_t cap, std::vector<ClassBuf *> &out)
 
 void class_free(ClassBuf *c)
 {
-    if (c->base) hipFree(c->base);
-    if (c->branch) hipStreamDestroy(c->branch);
-    if (c->join) hipEventDestroy(c->join);
-    if (c->ready) hipEventDestroy(c->ready);
+    if (c->base) (void) hipFree(c->base);
+    if (c->branch) (void) hipStreamDestroy(c->branch);
+    if (c->join) (void) hipEventDestroy(c->join);
+    if (c->ready) (void) hipEventDestroy(c->ready);
     delete c;
 }
 
@@ -167,7 +167,7 @@ int class_checkout(int device, int W, int H, int count, hipStream_t s, ClassBuf 
 void class_return(ClassBuf *c, hipStream_t s)
 {
     c->pending = hipEventRecord(c->ready, s) == hipSuccess;
-    if (!c->pending) hipStreamSynchronize(s);
+    if (!c->pending) (void) hipStreamSynchronize(s);   // no event: wait here instead
     std::vector<ClassBuf *> drop;
     {
         std::lock_guard<std::mutex> lk(g_mtx);
@@ -423,10 +423,10 @@ int run_batch(const imresh_image *imgs, uint32_t count, const imresh_params *par
     hip(es);
     for (auto &kv : classes)
         if (kv.second.buf) class_return(kv.second.buf, s);
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
-    if (weights) hipFreeAsync(weights, s);
-    hipEventDestroy(ev);
+    if (exec) (void) hipGraphExecDestroy(exec);
+    if (graph) (void) hipGraphDestroy(graph);
+    if (weights) (void) hipFreeAsync(weights, s);
+    (void) hipEventDestroy(ev);
     return st;
 }
 

@@ -97,8 +97,8 @@ int ws_alloc(int device, int W, int H, Workspace **out)
     ws_layout(w, W, H, base);
     e = hipHostMalloc((void **) &w->host_stage, 4096, hipHostMallocDefault);
     if (e != hipSuccess) {
-        hipFree(w->base);
-        if (w->host_stage) hipHostFree(w->host_stage);
+        (void) hipFree(w->base);
+        if (w->host_stage) (void) hipHostFree(w->host_stage);
         delete w;
         return from_hip(e);
     }
@@ -109,11 +109,11 @@ int ws_alloc(int device, int W, int H, Workspace **out)
 void ws_release(Workspace *w)
 {
     if (w == nullptr) return;
-    if (w->base) hipFree(w->base);
-    if (w->weights) hipFree(w->weights);
-    if (w->host_stage) hipHostFree(w->host_stage);
-    if (w->host_wpin) hipHostFree(w->host_wpin);
-    if (w->gexec) hipGraphExecDestroy(w->gexec);
+    if (w->base) (void) hipFree(w->base);
+    if (w->weights) (void) hipFree(w->weights);
+    if (w->host_stage) (void) hipHostFree(w->host_stage);
+    if (w->host_wpin) (void) hipHostFree(w->host_wpin);
+    if (w->gexec) (void) hipGraphExecDestroy(w->gexec);
     delete w;
 }
 
@@ -250,7 +250,7 @@ void ws_return(Workspace *w, hipStream_t s, bool synced = false)
     } else {
         w->busy.store(true, std::memory_order_relaxed);
         if (hipLaunchHostFunc(s, mark_idle, w) != hipSuccess) {
-            hipStreamSynchronize(s);
+            (void) hipStreamSynchronize(s);   // no host function: wait here instead
             w->busy.store(false, std::memory_order_release);
         }
     }
@@ -342,7 +342,7 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
     // workspaces are idle; this only guards)
     if (!ws_wait_idle(w, 10.0)) return IMRESH_ERR_INTERNAL;
     if (w->host_wpin_n < host.size()) {
-        if (w->host_wpin) hipHostFree(w->host_wpin);
+        if (w->host_wpin) (void) hipHostFree(w->host_wpin);
         w->host_wpin = nullptr;
         w->host_wpin_n = 0;
         IMR_HIP_CHECK(hipHostMalloc((void **) &w->host_wpin, sizeof(float) * host.size(), hipHostMallocDefault));
@@ -353,7 +353,7 @@ int ensure_weights(Workspace *w, const Params &q, hipStream_t s)
     if (w->wcap < rows) {
         if (w->weights) {
             IMR_HIP_CHECK(hipStreamSynchronize(s));
-            hipFree(w->weights);
+            (void) hipFree(w->weights);
             w->weights = nullptr;
         }
         w->wcap = rows;
@@ -402,17 +402,17 @@ int time_variant(bool row, int var, Plan2D p, Workspace *w, hipStream_t s, float
     for (int k = 0; k < trials && st == IMRESH_OK; ++k) {
         st = launch_fill_state(w->X, w->G, w->mask, w->M, w->Bn, w->n, w->W, s);
         for (int r = 0; r <= reps && st == IMRESH_OK; ++r) {
-            if (r == 1) hipEventRecord(e0, s);   // first launch warms up
-            st = row ? launch_row(ROW_MID, p, ra, s) : launch_col(col_mode, p, ca, s);
+            if (r == 1) st = IMR_HIP_STATUS(hipEventRecord(e0, s));   // first launch warms up
+            if (st == IMRESH_OK) st = row ? launch_row(ROW_MID, p, ra, s) : launch_col(col_mode, p, ca, s);
         }
-        hipEventRecord(e1, s);
+        if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventRecord(e1, s));
         if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventSynchronize(e1));
         float t = 0.f;
-        if (st == IMRESH_OK) hipEventElapsedTime(&t, e0, e1);
+        if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipEventElapsedTime(&t, e0, e1));
         best = std::min(best, t / reps);
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
     *ms = best;
     if (st == IMRESH_OK && std::getenv("IMRESH_AUTOTUNE_VERBOSE") != nullptr)
         std::fprintf(stderr, "[imresh autotune] %dx%d %s variant %d: %.2f us\n", p.W, p.H,
@@ -568,7 +568,7 @@ int cycle_graph(Workspace *w, const Params &q, const Plan2D &p)
         // replays of this graph by the workspace's previous user must have
         // run before it is destroyed (ws_checkout hands out idle workspaces)
         if (!ws_wait_idle(w, 10.0)) return IMRESH_ERR_INTERNAL;
-        hipGraphExecDestroy(w->gexec);
+        (void) hipGraphExecDestroy(w->gexec);
         w->gexec = nullptr;
     }
     IMR_CHECK(prepare_blur(w->wR));
@@ -580,12 +580,12 @@ int cycle_graph(Workspace *w, const Params &q, const Plan2D &p)
     const int st = enqueue_cycle(w, q, p, cs);
     const hipError_t e = hipStreamEndCapture(cs, &g);
     if (st != IMRESH_OK) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void) hipGraphDestroy(g);
         return st;
     }
     IMR_HIP_CHECK(e);
     const hipError_t ei = hipGraphInstantiate(&w->gexec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void) hipGraphDestroy(g);
     IMR_HIP_CHECK(ei);
     w->g_nhio = q.n_hio;
     w->g_beta = q.beta;
@@ -730,7 +730,7 @@ struct Scratch {
         if (b == nullptr) return;
         b->state.store(2, std::memory_order_relaxed);
         if (hipLaunchHostFunc(s, pool_buf_free, b) != hipSuccess) {
-            hipStreamSynchronize(s);
+            (void) hipStreamSynchronize(s);   // no host function: wait here instead
             b->state.store(0, std::memory_order_release);
         }
     }
@@ -751,7 +751,7 @@ void pool_bufs_release()
         g_pb.swap(keep);
     }
     for (PoolBuf *x : drop) {
-        hipFree(x->p);
+        (void) hipFree(x->p);
         delete x;
     }
 }
@@ -800,7 +800,7 @@ int reduce_slot_get(int dev, ReduceSlot *out)
     IMR_HIP_CHECK(hipMalloc(&r.dev, kReduceSlotBytes));
     const hipError_t e = hipHostMalloc((void **) &r.host, 64, hipHostMallocDefault);
     if (e != hipSuccess) {
-        hipFree(r.dev);
+        (void) hipFree(r.dev);
         return from_hip(e);
     }
     *out = r;
@@ -824,8 +824,8 @@ void reduce_slots_release()
     }
     for (auto &kv : drop)
         for (const ReduceSlot &r : kv.second) {
-            hipFree(r.dev);
-            hipHostFree(r.host);
+            (void) hipFree(r.dev);
+            (void) hipHostFree(r.host);
         }
 }
 
@@ -856,7 +856,7 @@ struct SlotHold {
     ~SlotHold()
     {
         if (!held) return;
-        if (!done) hipStreamSynchronize(s);
+        if (!done) (void) hipStreamSynchronize(s);   // release the slot only after its last use
         reduce_slot_put(dev, r);
     }
 };
@@ -1269,7 +1269,7 @@ static int debug_cycle_mask_impl(const float *modulus_host, uint32_t width, uint
     if (st == IMRESH_OK) st = recon_stage_results(w, s);
     if (st == IMRESH_OK) st = IMR_HIP_STATUS(hipStreamSynchronize(s));
     if (st == IMRESH_OK && threshold_host != nullptr) *threshold_host = w->host_stage[0];
-    if (st != IMRESH_OK) hipStreamSynchronize(s);
+    if (st != IMRESH_OK) (void) hipStreamSynchronize(s);   // drain; st is the status
     ws_return(w, s, true);
     return st;
 }
@@ -1305,7 +1305,8 @@ int run_hio_standalone(Workspace *w, unsigned n_cycles, float target, float beta
     {
         std::vector<float> field(n);
         srand(2623091912u);
-        for (size_t i = 0; i < n; ++i) field[i] = (float) rand() / RAND_MAX;
+        // RAND_MAX converts to float (2^31) as in the reference's expression
+        for (size_t i = 0; i < n; ++i) field[i] = (float) rand() / (float) RAND_MAX;
         IMR_HIP_CHECK(hipMemcpyAsync(w->B2, field.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
         IMR_HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -1757,13 +1758,14 @@ int imresh_profile_kernels(uint32_t W, uint32_t H, uint32_t reps, float *out_ms,
                 acc[k] += ms;
                 cnt[k] += 1;
             }
-            hipEventDestroy(ev[e]);
-            hipEventDestroy(ev[e + 1]);
+            (void) hipEventDestroy(ev[e]);
+            (void) hipEventDestroy(ev[e + 1]);
         }
         ev.clear();
     }
-    hipStreamSynchronize(s);
-    ws_return(w, s, true);
+    const int ss = IMR_HIP_STATUS(hipStreamSynchronize(s));
+    ws_return(w, s, ss == IMRESH_OK);
+    if (st == IMRESH_OK) st = ss;
     if (st == IMRESH_OK)
         for (int k = 0; k < 4; ++k) out_ms[k] = cnt[k] ? (float) (acc[k] / cnt[k]) : 0.f;
     return st;
@@ -1807,13 +1809,14 @@ int imresh_profile_forward_model(uint32_t W, uint32_t H, uint32_t reps, float *o
                     acc[k] += ms;
                     cnt[k] += 1;
                 }
-                if (a) hipEventDestroy(a);
-                if (b) hipEventDestroy(b);
+                if (a) (void) hipEventDestroy(a);
+                if (b) (void) hipEventDestroy(b);
             }
         }
     }
-    hipStreamSynchronize(s);
-    ws_return(w, s, true);
+    const int ss = IMR_HIP_STATUS(hipStreamSynchronize(s));
+    ws_return(w, s, ss == IMRESH_OK);
+    if (st == IMRESH_OK) st = ss;
     if (st == IMRESH_OK)
         for (int k = 0; k < 2; ++k) out_ms[k] = cnt[k] ? (float) (acc[k] / cnt[k]) : 0.f;
     return st;

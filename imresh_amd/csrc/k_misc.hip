@@ -420,7 +420,7 @@ template <int... Rs>
 static int prepare_blur_dispatch(int r, std::integer_sequence<int, Rs...>)
 {
     int st = IMRESH_ERR_INVALID;
-    ((r == Rs ? (st = prepare_blur_r<Rs>(), true) : false) || ...);
+    (void) ((r == Rs ? (st = prepare_blur_r<Rs>(), true) : false) || ...);
     return st;
 }
 
@@ -435,7 +435,7 @@ static int launch_blur_dispatch(int r, const BlurArgs &a, hipStream_t s,
                                 std::integer_sequence<int, Rs...>)
 {
     int st = IMRESH_ERR_INVALID;
-    ((r == Rs ? (st = launch_blur_r<Rs>(a, s), true) : false) || ...);
+    (void) ((r == Rs ? (st = launch_blur_r<Rs>(a, s), true) : false) || ...);
     return st;
 }
 

@@ -147,6 +147,30 @@ std::vector<float2> blue_tables(int n, int m)
     return out;
 }
 
+// copies a host table into a new allocation on `device`, leaving the calling
+// thread's current device as it was
+int upload_table(int device, const std::vector<float2> &v, float2 **out)
+{
+    *out = nullptr;
+    int prev = 0;
+    IMR_HIP_CHECK(hipGetDevice(&prev));
+    IMR_HIP_CHECK(hipSetDevice(device));
+    float2 *p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(float2) * v.size());
+    if (e == hipSuccess) {
+        e = hipMemcpy(p, v.data(), sizeof(float2) * v.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) (void) hipFree(p);
+    }
+    const hipError_t r = hipSetDevice(prev);
+    if (e != hipSuccess) return from_hip(e);
+    if (r != hipSuccess) {
+        (void) hipFree(p);
+        return from_hip(r);
+    }
+    *out = p;
+    return IMRESH_OK;
+}
+
 }  // namespace
 
 bool line_supported(int n) { return n >= 1 && n <= kMaxLine; }
@@ -167,15 +191,8 @@ int get_line_plan(int device, int n, LinePlan *out)
         if (m == 0) return IMRESH_ERR_UNSUPPORTED_SIZE;
         LinePlan lm;
         IMR_CHECK(get_line_plan(device, m, &lm));
-        const std::vector<float2> tab = blue_tables(n, m);
         float2 *dev = nullptr;
-        int prev = 0;
-        hipGetDevice(&prev);
-        hipSetDevice(device);
-        hipError_t e = hipMalloc(&dev, sizeof(float2) * tab.size());
-        if (e == hipSuccess) e = hipMemcpy(dev, tab.data(), sizeof(float2) * tab.size(), hipMemcpyHostToDevice);
-        hipSetDevice(prev);
-        if (e != hipSuccess) return from_hip(e);
+        IMR_CHECK(upload_table(device, blue_tables(n, m), &dev));
         LinePlan lp = lm;
         lp.n = n;
         lp.blue = 1;
@@ -196,13 +213,7 @@ int get_line_plan(int device, int n, LinePlan *out)
     append_twiddles(inv, tw);
     tw.push_back(make_float2(0.f, 0.f));   // never empty
     float2 *dev = nullptr;
-    int prev = 0;
-    hipGetDevice(&prev);
-    hipSetDevice(device);
-    hipError_t e = hipMalloc(&dev, sizeof(float2) * tw.size());
-    if (e == hipSuccess) e = hipMemcpy(dev, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
-    hipSetDevice(prev);
-    if (e != hipSuccess) return from_hip(e);
+    IMR_CHECK(upload_table(device, tw, &dev));
     LinePlan lp{};
     lp.n = n;
     lp.group = pi->group;
@@ -215,7 +226,6 @@ int get_line_plan(int device, int n, LinePlan *out)
         // may first run inside a graph capture, where no allocation or copy
         // may happen) and the twiddles exp(-2 pi i j / n) of its outer steps
         LinePlan sub;
-        hipSetDevice(prev);
         IMR_CHECK(get_line_plan(device, n / 8, &sub));
         std::vector<float2> t(n);
         for (int j = 0; j < n; ++j) {
@@ -223,11 +233,7 @@ int get_line_plan(int device, int n, LinePlan *out)
             t[j] = make_float2((float) std::cos(ang), (float) std::sin(ang));
         }
         float2 *fs = nullptr;
-        hipSetDevice(device);
-        e = hipMalloc(&fs, sizeof(float2) * t.size());
-        if (e == hipSuccess) e = hipMemcpy(fs, t.data(), sizeof(float2) * t.size(), hipMemcpyHostToDevice);
-        hipSetDevice(prev);
-        if (e != hipSuccess) return from_hip(e);
+        IMR_CHECK(upload_table(device, t, &fs));
         lp.fs_h2 = n / 8;
         lp.fs_tw = fs;
     }

@@ -98,7 +98,7 @@ struct PinnedPool {
     }
     void release()
     {
-        for (void *p : all) hipHostFree(p);
+        for (void *p : all) (void) hipHostFree(p);
         all.clear();
         free.clear();
     }
@@ -296,7 +296,7 @@ struct Server {
             }
             bool gone = false;
             if (r->status != IMRESH_OK) {
-                hipStreamSynchronize(r->lane);   // drain what was queued before the error
+                (void) hipStreamSynchronize(r->lane);   // drain what was queued before the error
                 gone = true;
             } else if (r->finishing && hipEventQuery(r->fin) == hipSuccess) {
                 gone = true;
@@ -322,7 +322,7 @@ struct Server {
 
     void loop()
     {
-        hipSetDevice(device);
+        (void) hipSetDevice(device);   // the creating thread's current device, a valid index
         for (;;) {
             {
                 std::unique_lock<std::mutex> lk(mtx);
@@ -367,22 +367,22 @@ static int server_create_impl(uint64_t hbm_budget_bytes, uint32_t max_active, ui
     for (uint32_t i = 0; i < nl; ++i) {
         hipStream_t st = nullptr;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-            for (hipStream_t x : s.lanes) hipStreamDestroy(x);
+            for (hipStream_t x : s.lanes) (void) hipStreamDestroy(x);
             delete h;
             return IMRESH_ERR_HIP;
         }
         s.lanes.push_back(st);
     }
     if (hipStreamCreateWithFlags(&s.tune, hipStreamNonBlocking) != hipSuccess) {
-        for (hipStream_t x : s.lanes) hipStreamDestroy(x);
+        for (hipStream_t x : s.lanes) (void) hipStreamDestroy(x);
         delete h;
         return IMRESH_ERR_HIP;
     }
     try {
         s.thr = std::thread([&s] { s.loop(); });
     } catch (...) {
-        for (hipStream_t x : s.lanes) hipStreamDestroy(x);
-        hipStreamDestroy(s.tune);
+        for (hipStream_t x : s.lanes) (void) hipStreamDestroy(x);
+        (void) hipStreamDestroy(s.tune);
         delete h;
         return IMRESH_ERR_INTERNAL;
     }
@@ -490,12 +490,12 @@ int imresh_server_destroy(imresh_server *h)
     }
     s.cv.notify_all();
     if (s.thr.joinable()) s.thr.join();
-    for (hipStream_t st : s.lanes) hipStreamSynchronize(st);
+    for (hipStream_t st : s.lanes) (void) hipStreamSynchronize(st);
     for (auto &kv : s.idle)
         for (imr::Workspace *w : kv.second) imr::ws_release(w);
-    for (hipEvent_t e : s.event_pool) hipEventDestroy(e);
-    for (hipStream_t st : s.lanes) hipStreamDestroy(st);
-    hipStreamDestroy(s.tune);
+    for (hipEvent_t e : s.event_pool) (void) hipEventDestroy(e);
+    for (hipStream_t st : s.lanes) (void) hipStreamDestroy(st);
+    (void) hipStreamDestroy(s.tune);
     s.pinned.release();
     delete h;
     return IMRESH_OK;

@@ -60,7 +60,7 @@ void worker_main(size_t idx)
 {
     t_queue_worker = true;
     const Slot &slot = g_q.slots[idx];
-    hipSetDevice(slot.device);
+    (void) hipSetDevice(slot.device);   // selected once already in queue_init
     for (;;) {
         Task t;
         {
@@ -92,9 +92,9 @@ int queue_init(int per_device)
     // streams on every device; on a failure those created so far are
     // destroyed and the queue stays uninitialized
     auto unwind = [&](hipError_t err) {
-        for (auto &s : g_q.slots) hipStreamDestroy(s.stream);
+        for (auto &s : g_q.slots) (void) hipStreamDestroy(s.stream);
         g_q.slots.clear();
-        hipSetDevice(caller_dev);
+        (void) hipSetDevice(caller_dev);   // restore only; err is the status
         return imr::from_hip(err);
     };
     for (int d = 0; d < ndev; ++d) {
@@ -108,7 +108,7 @@ int queue_init(int per_device)
             g_q.slots.push_back(std::move(s));
         }
     }
-    hipSetDevice(caller_dev);   // the caller's current device is left as it was
+    (void) hipSetDevice(caller_dev);   // the caller's current device is left as it was
     g_q.capacity = g_q.slots.size();
     g_q.running = true;
     for (size_t i = 0; i < g_q.slots.size(); ++i) g_q.slots[i].worker = std::thread(worker_main, i);
