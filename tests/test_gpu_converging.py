@@ -2,11 +2,12 @@
 (GPU).
 
 tests/golden/converging.npz (make_converging.py) holds the float64 oracle's
-end results of five workloads of the benchmark's object generator (256^2
+end results of six workloads of the benchmark's object generator (256^2
 with one and two shapes, 2048^2 with two shapes, and 300^2 and 1000^2 with
 one shape -- lengths without a compiled plan, Bluestein transforms on the
-GPU; no Poisson noise) on which shrink-wrap converges to the known object in
-every precision, so the end result no longer depends on rounding.  The GPU reconstruction of the same
+GPU -- and 512^2 with sigma0 = 6, the 35-tap blur; no Poisson noise) on
+which shrink-wrap converges to the known object in every precision, so the
+end result no longer depends on rounding.  The GPU reconstruction of the same
 modulus must match it:
 
 * NCC to the known object within 1e-3 of the float64 oracle's (the
@@ -35,7 +36,7 @@ from imresh_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 FIX = os.path.join(os.path.dirname(__file__), "golden", "converging.npz")
-NAMES = ["c256a", "c256b", "c2048", "b300", "b1000"]
+NAMES = ["c256a", "c256b", "c2048", "b300", "b1000", "c512s6"]
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -44,7 +45,7 @@ def test_full_reconstruction_matches_float64_oracle(name):
     n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
     obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
     x = M.copy()
-    r = im.shrink_wrap(x, n_cycles=20, target_error=FLT_MIN, n_hio_cycles=20)
+    r = im.shrink_wrap(x, n_cycles=20, target_error=FLT_MIN, n_hio_cycles=20, sigma0=float(z[f"{name}_sigma0"]))
     assert r.cycles_run == 20
     e64, e32, e1 = float(z[f"{name}_err64"]), float(z[f"{name}_err32"]), float(z[f"{name}_err64_c1"])
     n64 = float(z[f"{name}_ncc64"])

@@ -212,7 +212,8 @@ def test_converging_fixture_float32_within_the_gpu_bounds(name):
     oracle.set_fft_threads(8)
     n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
     obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
-    kw = dict(n_cycles=20, target_error=np.finfo(np.float32).tiny, n_hio_cycles=20)
+    kw = dict(n_cycles=20, target_error=np.finfo(np.float32).tiny, n_hio_cycles=20,
+              sigma0=float(z[f"{name}_sigma0"]))
     x64, s64 = oracle.shrink_wrap(M, dtype=np.float64, **kw)
     y0, x0, h, w = (int(v) for v in z[f"{name}_box"])
     st = int(z[f"{name}_step"])
