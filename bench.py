@@ -154,7 +154,7 @@ def parity_check(L, M: np.ndarray) -> dict:
     z = np.load(os.path.join(ROOT, "tests", "golden", "converging.npz"), allow_pickle=False)
     for name, key in (("c256a", "converging_256"), ("b300", "converging_300_bluestein")):
         n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
-        obj, Mc = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
+        obj, Mc = syn.workload(n, int(z[f"{name}_height"]), n_shapes=shapes, central=central, index=idx)
         x = Mc.copy()
         r = im.shrink_wrap(x, n_cycles=N_CYCLES, target_error=FLT_MIN, n_hio_cycles=N_HIO)
         y0, x0, h, w = (int(v) for v in z[f"{name}_box"])

@@ -2,12 +2,13 @@
 (GPU).
 
 tests/golden/converging.npz (make_converging.py) holds the float64 oracle's
-end results of six workloads of the benchmark's object generator (256^2
+end results of eight workloads of the benchmark's object generator (256^2
 with one and two shapes, 2048^2 with two shapes, and 300^2 and 1000^2 with
 one shape -- lengths without a compiled plan, Bluestein transforms on the
-GPU -- and 512^2 with sigma0 = 6, the 35-tap blur; no Poisson noise) on
-which shrink-wrap converges to the known object in every precision, so the
-end result no longer depends on rounding.  The GPU reconstruction of the same
+GPU --, 512^2 with sigma0 = 6, the 35-tap blur, and 8192 x 64 and
+64 x 8192, the 8192-point row and four-step column passes; no Poisson noise)
+on which shrink-wrap converges to the known object in every precision, so
+the end result no longer depends on rounding.  The GPU reconstruction of the same
 modulus must match it:
 
 * NCC to the known object within 1e-3 of the float64 oracle's (the
@@ -36,14 +37,15 @@ from imresh_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 FIX = os.path.join(os.path.dirname(__file__), "golden", "converging.npz")
-NAMES = ["c256a", "c256b", "c2048", "b300", "b1000", "c512s6"]
+NAMES = ["c256a", "c256b", "c2048", "b300", "b1000", "c512s6", "w8192", "h8192"]
 
 
 @pytest.mark.parametrize("name", NAMES)
 def test_full_reconstruction_matches_float64_oracle(name):
     z = np.load(FIX, allow_pickle=False)
     n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
-    obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
+    hgt = int(z[f"{name}_height"])
+    obj, M = syn.workload(n, hgt, n_shapes=shapes, central=central, index=idx)
     x = M.copy()
     r = im.shrink_wrap(x, n_cycles=20, target_error=FLT_MIN, n_hio_cycles=20, sigma0=float(z[f"{name}_sigma0"]))
     assert r.cycles_run == 20
@@ -63,7 +65,7 @@ def test_full_reconstruction_matches_float64_oracle(name):
     l1 = float(np.abs(x.astype(np.float64)).sum())
     share_gpu = (r.last_error * r.last_masked) ** 2 / l1
     share64 = (e64 * masked64) ** 2 / l1
-    print(f"{name} ({n}^2, {shapes} shapes): NCC gpu {ncc:.9f} fp64 {n64:.9f}; object rel L2 vs fp64 "
+    print(f"{name} ({n}x{hgt}, {shapes} shapes): NCC gpu {ncc:.9f} fp64 {n64:.9f}; object rel L2 vs fp64 "
           f"{rel:.2e}; outside-box norm gpu {out_gpu / norm64:.2e} fp64 {out64 / norm64:.2e} (of the "
           f"image norm); HIO error gpu {r.last_error:.7g} fp64 {e64:.7g} fp32 {e32:.7g} first cycle "
           f"{e1:.5g}; residual share of the L1 mass gpu {share_gpu:.3e} fp64 {share64:.3e}; masked "

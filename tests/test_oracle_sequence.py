@@ -211,7 +211,7 @@ def test_converging_fixture_float32_within_the_gpu_bounds(name):
     z = np.load(os.path.join(os.path.dirname(__file__), "golden", "converging.npz"), allow_pickle=False)
     oracle.set_fft_threads(8)
     n, shapes, central, idx = (int(v) for v in z[f"{name}_spec"])
-    obj, M = syn.workload(n, n, n_shapes=shapes, central=central, index=idx)
+    obj, M = syn.workload(n, int(z[f"{name}_height"]), n_shapes=shapes, central=central, index=idx)
     kw = dict(n_cycles=20, target_error=np.finfo(np.float32).tiny, n_hio_cycles=20,
               sigma0=float(z[f"{name}_sigma0"]))
     x64, s64 = oracle.shrink_wrap(M, dtype=np.float64, **kw)
