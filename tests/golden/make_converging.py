@@ -44,7 +44,8 @@ FLT_MIN = float(np.finfo(np.float32).tiny)
 # precision); b300 and b1000 are lengths without a compiled plan (Bluestein
 # transforms of 1024 and 2048 points on the GPU); c512s6 runs the 35-tap
 # blur of sigma0 = 6; w8192 has 8192-point rows and h8192 8192-point columns
-# (the four-step column pass).  Reconstructions that span the whole image
+# (the four-step column pass); b3000 and b5000 have lines convolved over 8192
+# and 16384 points.  Reconstructions that span the whole image
 # (the object lands across its edges) keep every step-th row and column of
 # the crop.
 WORKLOADS = {
@@ -56,6 +57,8 @@ WORKLOADS = {
     "c512s6": (512, 512, 1, 256, 102, 2, 6.0),
     "w8192": (8192, 64, 1, 32, 100, 4, 3.0),
     "h8192": (64, 8192, 1, 32, 100, 4, 3.0),
+    "b3000": (3000, 64, 1, 32, 100, 4, 3.0),
+    "b5000": (64, 5000, 1, 32, 100, 4, 3.0),
 }
 
 

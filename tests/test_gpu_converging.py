@@ -2,11 +2,12 @@
 (GPU).
 
 tests/golden/converging.npz (make_converging.py) holds the float64 oracle's
-end results of eight workloads of the benchmark's object generator (256^2
+end results of ten workloads of the benchmark's object generator (256^2
 with one and two shapes, 2048^2 with two shapes, and 300^2 and 1000^2 with
 one shape -- lengths without a compiled plan, Bluestein transforms on the
 GPU --, 512^2 with sigma0 = 6, the 35-tap blur, and 8192 x 64 and
-64 x 8192, the 8192-point row and four-step column passes; no Poisson noise)
+64 x 8192, the 8192-point row and four-step column passes, and 3000 x 64
+and 64 x 5000, Bluestein lines over 8192 and 16384 points; no Poisson noise)
 on which shrink-wrap converges to the known object in every precision, so
 the end result no longer depends on rounding.  The GPU reconstruction of the same
 modulus must match it:
@@ -37,7 +38,7 @@ from imresh_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 FIX = os.path.join(os.path.dirname(__file__), "golden", "converging.npz")
-NAMES = ["c256a", "c256b", "c2048", "b300", "b1000", "c512s6", "w8192", "h8192"]
+NAMES = ["c256a", "c256b", "c2048", "b300", "b1000", "c512s6", "w8192", "h8192", "b3000", "b5000"]
 
 
 @pytest.mark.parametrize("name", NAMES)
