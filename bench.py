@@ -379,6 +379,42 @@ def _percentiles(lat):
             "p99.9_ms": round(float(np.percentile(a, 99.9)), 2), "max_ms": round(float(a[-1]), 2)}
 
 
+def serve_plain_loops(trace, mods, server_ios) -> dict:
+    """The plain per-image GPU loop over the serving requests, every request
+    available at t=0 (the counterpart of the server's capacity run): moduli
+    resident on the device, one imresh_shrinkwrap_device call per request,
+    either one image at a time on one stream or round-robin on 3 streams
+    (the batched leg's plain loop).  Outputs are compared with the server's
+    bit for bit."""
+    count = len(trace)
+    params = [im.default_params(n_cycles=c, n_hio_cycles=N_HIO, target_error=FLT_MIN) for _, _, c in trace]
+    bufs = [im.DeviceBuffer.from_array(m) for m in mods]
+    out = {"inputs": "device-resident moduli, all requests available at t=0"}
+    same = True
+    for name, nst in (("one_stream", 1), ("three_streams", 3)):
+        for b, m in zip(bufs, mods):
+            b.upload(m)
+        streams = [im.stream_create() for _ in range(nst)]
+        im.sync()
+        t0 = time.perf_counter()
+        for k, (b, m) in enumerate(zip(bufs, mods)):
+            im.shrink_wrap_device(b.ptr, m.shape[1], m.shape[0], params[k], stream=streams[k % nst],
+                                  want_result=False)
+        for st in streams:
+            im.sync(st)
+        dt = time.perf_counter() - t0
+        for st in streams:
+            im.stream_destroy(st)
+        out[name] = {"recon_per_s": round(count / dt, 2), "streams": nst}
+        same = same and all(np.array_equal(b.download(np.float32, m.shape), x)
+                            for b, m, x in zip(bufs, mods, server_ios))
+    for b in bufs:
+        b.close()
+    out["outputs_bitwise_equal_to_server"] = bool(same)
+    log(f"[bench] serving plain loops: {out}")
+    return out
+
+
 def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, modes=None) -> dict:
     """Serving workload: `count` requests of synthetic.serve_trace (Poisson
     arrivals at `rate`/s, sizes log-uniform in [128, 2048], 5/10/20 cycles x
@@ -476,6 +512,7 @@ def serve_bench(L, trace, mods, rate: float = 200.0, budget_gib: int = 200, mode
            "outputs_bitwise_equal": bool(same)}
     out["speedup_recon_per_s"] = round(out["server"]["recon_per_s"] / out["task_queue"]["recon_per_s"], 3)
     log(f"[bench] serving: server {out['server']}, task queue {out['task_queue']}, equal {same}")
+    serve_bench.last_server_ios = ios_s
     return out
 
 
@@ -791,6 +828,11 @@ def main() -> int:
         log("[bench] serving capacity: the same requests all at once")
         serve["capacity"] = serve_bench(L, [(0.0, n, c) for _, n, c in trace], smods, rate=float("inf"),
                                         modes=(0,))
+        cap = serve["capacity"]
+        cap["plain_loop"] = serve_plain_loops(trace, smods, serve_bench.last_server_ios)
+        for name in ("one_stream", "three_streams"):
+            cap[f"speedup_vs_plain_loop_{name}"] = round(
+                cap["server"]["recon_per_s"] / cap["plain_loop"][name]["recon_per_s"], 3)
         log("[bench] serving steady state (1000 requests at 100/s)")
         serve["steady_state"] = serve_bench(L, light, lmods, rate=100.0, modes=(0,))
 
