@@ -92,3 +92,57 @@ def test_pooled_state_outlives_destroyed_streams():
                 assert np.array_equal(b.download(np.float32, m.shape), r)
     finally:
         im.set_option(im.OPT_CYCLE_GRAPHS, 1)
+
+
+def test_concurrent_reconstructions_from_host_threads():
+    """Eight host threads reconstruct 32 images of eight sizes no other test
+    uses, half through the host entry point (default stream) and half
+    through the device entry point on a stream of their own, so plans,
+    Bluestein tables, tuned variants and pooled workspaces of a size are
+    first created while other threads ask for them.  Every result must
+    equal the same reconstruction run alone afterwards, bit for bit."""
+    import ctypes
+    import threading
+
+    sizes = [(77, 91), (131, 263), (389, 157), (509, 97), (1031, 61), (61, 1031), (200, 333), (96, 97)]
+    mods = [syn.workload(w, h, n_shapes=4, central=min(w, h) // 2, index=300 + i)[1]
+            for i, (w, h) in enumerate(sizes)]
+    p = im.default_params(n_cycles=2, n_hio_cycles=5, target_error=FLT_MIN)
+    jobs = [j % len(sizes) for j in range(32)]
+    out, errors = [None] * len(jobs), []
+
+    def work(tid):
+        s = im.stream_create()
+        try:
+            for j in range(tid, len(jobs), 8):
+                m = mods[jobs[j]]
+                h, w = m.shape
+                if j % 2 == 0:
+                    x = m.copy()
+                    st = im.lib().imresh_shrinkwrap(x.ctypes.data, w, h, ctypes.byref(p), None, None)
+                    out[j] = x
+                else:
+                    d = im.DeviceBuffer.from_array(m)
+                    st = im.lib().imresh_shrinkwrap_device(d.ptr, w, h, ctypes.byref(p), s, None)
+                    if st == 0:
+                        im.sync(s)
+                        out[j] = d.download(np.float32, m.shape)
+                if st != 0:
+                    errors.append((j, sizes[jobs[j]], st))
+        finally:
+            im.stream_destroy(s)
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
+    for i, m in enumerate(mods):
+        x = m.copy()
+        h, w = m.shape
+        im.check(im.lib().imresh_shrinkwrap(x.ctypes.data, w, h, ctypes.byref(p), None, None))
+        for j, k in enumerate(jobs):
+            if k == i:
+                assert np.array_equal(out[j], x), (j, sizes[i])
