@@ -105,6 +105,7 @@ struct BlurArgs {
     // non-NULL: the weights of image b are wh/wv + kBlurWeightRow * cycles[b],
     // cycles[b] = completed shrink-wrap cycles (one weights row per cycle)
     const unsigned *cycles;
+    int xcd_tiles;         // set by launch_blur: tiles in contiguous ranges per XCD
 };
 
 // per-image device scalars of a batch: kScalStride floats {threshold, error,

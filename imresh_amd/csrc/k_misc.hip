@@ -40,7 +40,22 @@ __global__ void __launch_bounds__(256) k_blur2d(BlurArgs a)
     float *A = reinterpret_cast<float *>(blur_sm);
     __shared__ float red[4];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, W = a.W, H = a.H;
+    // XCD-aware tile order (a.xcd_tiles): workgroups are dealt round-robin
+    // to the 8 XCDs (each with its own L2), so row-major tiles put every
+    // horizontal neighbour on another XCD and fetch the 128-byte lines their
+    // halos share twice; with xcd_tiles each XCD takes a contiguous range of
+    // tiles instead (launch_blur sets it where the planes stay in the
+    // Infinity Cache)
+    int tx = blockIdx.x, ty = blockIdx.y;
+    {
+        const int gx = gridDim.x, nt = gridDim.x * gridDim.y;
+        if (a.xcd_tiles && (nt & 7) == 0) {
+            const int lin = tx + ty * gx, k = (lin & 7) * (nt >> 3) + (lin >> 3);
+            tx = k % gx;
+            ty = k / gx;
+        }
+    }
+    const int x0 = tx * TX, y0 = ty * TY, W = a.W, H = a.H;
     const size_t wrow = a.cycles != nullptr ? (size_t) kBlurWeightRow * a.cycles[b * kFlagStride] : 0;
     const float *in = a.in + (size_t) b * a.istride;
     float *out = a.out + (size_t) b * a.istride;
@@ -132,7 +147,7 @@ __global__ void __launch_bounds__(256) k_blur2d(BlurArgs a)
         if (lane == 0) red[wave] = lmax;
         __syncthreads();
         if (t == 0)
-            a.pmax[((size_t) b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+            a.pmax[((size_t) b * gridDim.y + ty) * gridDim.x + tx] =
                 fmax_ref(fmax_ref(red[0], red[1]), fmax_ref(red[2], red[3]));
     }
 }
@@ -446,7 +461,14 @@ int launch_blur(const BlurArgs &a, hipStream_t s)
     const int r = a.h_identity ? a.rv : a.v_identity ? a.rh : a.rh;
     if (!a.h_identity && !a.v_identity && a.rh != a.rv) return IMRESH_ERR_INVALID;
     if (r < 0 || r > kMaxBlurRadius || a.W < 1 || a.H < 1) return IMRESH_ERR_INVALID;
-    return launch_blur_dispatch(r, a, s, std::make_integer_sequence<int, kMaxBlurRadius + 1>{});
+    // XCD-contiguous tiles while the launch's planes stay in the Infinity
+    // Cache (measured: 1024^2 8.4 -> 7.8 us, 2048^2 17.2 -> 14.8 us, 4096^2
+    // 44.8 -> 43.5 us); from DRAM the row-major order keeps the XCDs reading
+    // neighbouring rows and is faster (8192^2: 155 against 166 us)
+    BlurArgs b = a;
+    const size_t elems = (size_t) a.W * a.H * (size_t) (a.batch > 1 ? a.batch : 1);
+    b.xcd_tiles = elems <= ((size_t) 16 << 20);
+    return launch_blur_dispatch(r, b, s, std::make_integer_sequence<int, kMaxBlurRadius + 1>{});
 }
 
 int launch_blur_h_inplace(float *d, int W, int H, const float *w, int r, hipStream_t s)
