@@ -35,10 +35,26 @@ enum ColKernel { CK_FWD, CK_INV, CK_PROJ, CK_FWD_REAL, CK_INV_REAL };
 template <class P, int C, bool SPLIT, int K, bool PF = false, bool BL = false, int STG = 0>
 __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
 {
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs
+    // (workgroup x + y * gridDim.x to XCD (x + y * gridDim.x) mod 8), each
+    // with its own L2; give every XCD a contiguous range of tiles so
+    // neighbouring tiles (which share 128-byte lines of X and M) meet in the
+    // same L2.  A grid whose width is a multiple of 8 keeps its images in
+    // place and remaps within each row of tiles; any other grid is remapped
+    // as a whole (consecutive tiles, of one image and the next, per XCD).
+    int bid = blockIdx.x, by = blockIdx.y;
+    if ((gridDim.x & 7) == 0) {
+        bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
+    } else {
+        const int gx = gridDim.x, nt = gx * gridDim.y, lin = bid + by * gx;
+        const int x8 = lin & 7, k = x8 * (nt >> 3) + min(x8, nt & 7) + (lin >> 3);
+        bid = k % gx;
+        by = k / gx;
+    }
     // image b of a batch and, for the middle step of a four-step column
     // pass, its sub-block sb (grid.y = images x sub)
     const int sub = a.sub > 1 ? a.sub : 1;
-    const int b = (int) blockIdx.y / sub, sb = (int) blockIdx.y - b * sub;
+    const int b = by / sub, sb = by - b * sub;
     if (a.done != nullptr && a.done[b * kFlagStride]) return;
     if constexpr (STG > 0) {
         // phase offset: the second half of the grid shares CUs with the first
@@ -50,12 +66,6 @@ __global__ void __launch_bounds__(C *P::T) k_col(ColArgs a)
     }
     extern __shared__ cf smem[];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
-    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
-    // each with its own L2; give every XCD a contiguous range of tiles so
-    // neighbouring tiles (which share 128-byte lines of X and M) meet in the
-    // same L2.
-    int bid = blockIdx.x;
-    if ((gridDim.x & 7) == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
     const int col = bid * C + c;
     const bool valid = col < a.W;
     const int W = a.W;
