@@ -29,7 +29,21 @@ namespace imr {
 template <int R>
 __global__ void __launch_bounds__(256) k_blur2d(BlurArgs a)
 {
-    const int b = blockIdx.z;   // image of a batch
+    // XCD-aware tile order (a.xcd_tiles): workgroups are dealt round-robin
+    // to the 8 XCDs (each with its own L2) in the order x + gx (y + gy z), so
+    // row-major tiles put every horizontal neighbour on another XCD and fetch
+    // the 128-byte lines their halos share twice; with xcd_tiles each XCD
+    // takes a contiguous range of the grid's tiles instead (launch_blur sets
+    // it where the planes stay in the Infinity Cache)
+    int tx = blockIdx.x, ty = blockIdx.y, b = blockIdx.z;   // b: image of a batch
+    if (a.xcd_tiles) {
+        const int gx = gridDim.x, gy = gridDim.y, nt = gx * gy * (int) gridDim.z;
+        const int lin = tx + gx * (ty + gy * b), x8 = lin & 7;
+        const int k = x8 * (nt >> 3) + min(x8, nt & 7) + (lin >> 3);
+        tx = k % gx;
+        ty = (k / gx) % gy;
+        b = k / (gx * gy);
+    }
     if (a.done != nullptr && a.done[b * kFlagStride]) return;
     constexpr int TX = kBlurTX, TY = kBlurTY, NT = 2 * R + 1;
     constexpr int AW = TX + 2 * R, AH = TY + 2 * R, AWP = (AW + 3) & ~3;
@@ -40,21 +54,6 @@ __global__ void __launch_bounds__(256) k_blur2d(BlurArgs a)
     float *A = reinterpret_cast<float *>(blur_sm);
     __shared__ float red[4];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    // XCD-aware tile order (a.xcd_tiles): workgroups are dealt round-robin
-    // to the 8 XCDs (each with its own L2), so row-major tiles put every
-    // horizontal neighbour on another XCD and fetch the 128-byte lines their
-    // halos share twice; with xcd_tiles each XCD takes a contiguous range of
-    // tiles instead (launch_blur sets it where the planes stay in the
-    // Infinity Cache)
-    int tx = blockIdx.x, ty = blockIdx.y;
-    {
-        const int gx = gridDim.x, nt = gridDim.x * gridDim.y;
-        if (a.xcd_tiles && (nt & 7) == 0) {
-            const int lin = tx + ty * gx, k = (lin & 7) * (nt >> 3) + (lin >> 3);
-            tx = k % gx;
-            ty = k / gx;
-        }
-    }
     const int x0 = tx * TX, y0 = ty * TY, W = a.W, H = a.H;
     const size_t wrow = a.cycles != nullptr ? (size_t) kBlurWeightRow * a.cycles[b * kFlagStride] : 0;
     const float *in = a.in + (size_t) b * a.istride;
