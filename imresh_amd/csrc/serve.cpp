@@ -106,6 +106,12 @@ struct PinnedPool {
 
 }  // namespace
 
+struct Server;
+// the server whose scheduler thread this is (completion callbacks run there):
+// a drain or destroy of that server from a callback would wait for the very
+// thread running it
+thread_local const Server *t_scheduler_of = nullptr;
+
 struct Server {
     int device = 0;
     uint64_t budget = 0;
@@ -323,6 +329,7 @@ struct Server {
     void loop()
     {
         (void) hipSetDevice(device);   // the creating thread's current device, a valid index
+        t_scheduler_of = this;
         for (;;) {
             {
                 std::unique_lock<std::mutex> lk(mtx);
@@ -459,6 +466,7 @@ int imresh_server_drain(imresh_server *h)
 {
     if (h == nullptr) return IMRESH_ERR_NULL;
     Server &s = h->s;
+    if (imr::t_scheduler_of == &s) return IMRESH_ERR_INVALID;   // from a completion callback
     std::unique_lock<std::mutex> lk(s.mtx);
     s.cv_idle.wait(lk, [&] { return s.completed == s.submitted; });
     return IMRESH_OK;
@@ -483,6 +491,7 @@ int imresh_server_destroy(imresh_server *h)
 {
     if (h == nullptr) return IMRESH_ERR_NULL;
     Server &s = h->s;
+    if (imr::t_scheduler_of == &s) return IMRESH_ERR_INVALID;   // from a completion callback
     imresh_server_drain(h);
     {
         std::lock_guard<std::mutex> lk(s.mtx);

@@ -208,7 +208,10 @@ typedef struct imresh_server_info {
 } imresh_server_info;
 
 /* Called on the server's scheduler thread when a request completes; io is
- * the request's buffer (Re(g') written).  Must not block for long. */
+ * the request's buffer (Re(g') written).  Must not block for long.  It may
+ * submit, poll and read stats; imresh_server_drain and imresh_server_destroy
+ * of the same server return IMRESH_ERR_INVALID there (they would wait for
+ * the thread running the callback). */
 typedef void (*imresh_server_callback)(void *user, const imresh_server_completion *c, float *io);
 
 /* Continuous batching on the current device: requests of any supported

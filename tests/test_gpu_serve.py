@@ -101,3 +101,35 @@ def test_server_many_small_requests_complete():
     assert sorted(comp) == sorted(ids)
     for (M, p), io in list(zip(reqs, ios))[::7]:
         assert np.array_equal(io, _alone(M, p)[0])
+
+
+def test_drain_and_destroy_from_a_callback_are_refused():
+    """Completion callbacks run on the server's scheduler thread: a drain or
+    destroy of the same server there would wait for that very thread, so both
+    return IMRESH_ERR_INVALID (2) and the server keeps running; submit, poll
+    and stats from the callback work."""
+    import ctypes
+    L = im.lib()
+    got = []
+    M = syn.workload(64, 64, n_shapes=4, central=32, index=77)[1]
+    p = im.default_params(n_cycles=1, n_hio_cycles=2, target_error=FLT_MIN)
+    with im.Server(max_active=4, lanes=2) as srv:
+        h = srv._h
+        info = im.ServerInfo()
+
+        @im.SERVER_CALLBACK
+        def cb(user, comp, io):
+            got.append((L.imresh_server_drain(h), L.imresh_server_destroy(h),
+                        L.imresh_server_stats(h, ctypes.byref(info))))
+
+        x = M.copy()
+        rid = ctypes.c_uint64()
+        im.check(L.imresh_server_submit(h, x.ctypes.data, 64, 64, ctypes.byref(p), 0, cb, None,
+                                        ctypes.byref(rid)))
+        srv.drain()
+        assert got == [(2, 2, 0)]
+        assert np.array_equal(x, _alone(M, p)[0])
+        y = M.copy()                           # still serving after the refused calls
+        srv.submit(y, p)
+        srv.drain()
+        assert np.array_equal(y, x)
