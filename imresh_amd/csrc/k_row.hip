@@ -206,14 +206,22 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
 // Row variants of a length (index: workgroup shape, then +count/2 for the
 // prefetching RK_MID): shapes 0 = E 16 with ~256-thread groups; for long
 // rows also 1 = E 32 (half the threads per row), 2 = E 16 with twice the
-// rows and 3 = shape 0 with the second half of the grid started ~3.4 us late.
+// rows and 3 = shape 0 with the second half of the grid started ~3.4 us late;
+// for short lines (<= 512 points, >= 8 rows per group) 1 = a quarter of the
+// rows per group: 4x the workgroups, which spreads one small image over more
+// CUs (a 256^2 reconstruction 199 -> 227 per second).
+template <int N>
+constexpr bool long_pow2() { return N >= 1024 && (N & (N - 1)) == 0; }
+
 template <int N, int... Rs>
 constexpr int row_shape_count()
 {
-    if constexpr (N >= 1024 && (N & (N - 1)) == 0) {   // extra shapes for the power-of-two lengths
-        using P16 = PlanE<16, N, Rs...>;
+    using P16 = PlanE<16, N, Rs...>;
+    if constexpr (long_pow2<N>()) {   // extra shapes for the long power-of-two lengths
         return (2 * P16::L * P16::T <= 1024 &&
                 sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 4 : 2;
+    } else if constexpr (N <= 512 && P16::L >= 8) {
+        return 2;   // short lines: also groups of a quarter of the rows (4x the workgroups)
     }
     return 1;
 }
@@ -259,18 +267,21 @@ int launch_row_direct(int var, int mode, RowArgs a, hipStream_t s)
     constexpr int ns = row_shape_count<N, Rs...>();
     const bool pf = var >= ns;
     const int shape = pf ? var - ns : var;
-    if constexpr (row_shape_count<N, Rs...>() > 1) {
+    if constexpr (long_pow2<N>() && ns > 1) {
         using P32 = PlanE<32, N, Rs...>;
         if (shape == 1) return launch_pf<P32, P32::L>(pf, mode, a, s);
         if constexpr (ns == 4) {
             if (shape == 2) return launch_pf<P16, 2 * P16::L>(pf, mode, a, s);
             if (shape == 3) return launch_pf<P16, P16::L, 1>(pf, mode, a, s);
         }
+    } else if constexpr (ns > 1) {
+        if (shape == 1) return launch_pf<P16, P16::L / 4>(pf, mode, a, s);
     }
     return launch_pf<P16, P16::L>(pf, mode, a, s);
 }
 
-// rows per workgroup of a variant (the host needs it for the partial sums)
+// rows per workgroup of a variant (the host needs it for the partial sums;
+// its buffers hold one slot per row, enough for any variant)
 template <int N, int... Rs>
 int row_variant_rows(int var)
 {
@@ -279,10 +290,12 @@ int row_variant_rows(int var)
     if (var >= kBlueVariant) return P16::L;
     constexpr int ns = row_shape_count<N, Rs...>();
     const int shape = var >= ns ? var - ns : var;
-    if constexpr (row_shape_count<N, Rs...>() > 1) {
+    if constexpr (long_pow2<N>() && ns > 1) {
         using P32 = PlanE<32, N, Rs...>;
         if (shape == 1) return P32::L;
         if (shape == 2) return 2 * P16::L;
+    } else if constexpr (ns > 1) {
+        if (shape == 1) return P16::L / 4;
     }
     return P16::L;
 }

@@ -46,7 +46,8 @@ def _gpu_iterations(M, g, gp, mask, beta, n_iter):
     return dg.download(np.complex64, (h, w)), dgp.download(np.complex64, (h, w))
 
 
-@pytest.mark.parametrize("w,h", [(2048, 2048), (1024, 1024), (4096, 256), (256, 2048), (72, 8192), (8192, 32)])
+@pytest.mark.parametrize("w,h", [(2048, 2048), (1024, 1024), (4096, 256), (256, 2048), (72, 8192), (8192, 32),
+                                 (256, 256), (200, 128)])
 def test_every_kernel_variant_computes_the_same_bits(w, h):
     """The autotuner picks among row/column kernel variants that differ only
     in tiling and scheduling (threads per line, lines per workgroup,
