@@ -206,20 +206,32 @@ int launch_variant(int mode, RowArgs a, hipStream_t s)
 // Row variants of a length (index: workgroup shape, then +count/2 for the
 // prefetching RK_MID): shapes 0 = E 16 with ~256-thread groups; for long
 // rows also 1 = E 32 (half the threads per row), 2 = E 16 with twice the
-// rows and 3 = shape 0 with the second half of the grid started ~3.4 us late;
+// rows, 3 = shape 0 with the second half of the grid started ~3.4 us late and,
+// when a group holds 2 or more rows, a last shape with half the rows: fewer
+// waves share each barrier, so the groups of a CU drift out of phase and
+// overlap one's loads with another's arithmetic (2048^2: 23.8 -> 22.6 us);
 // for short lines (<= 512 points, >= 8 rows per group) 1 = a quarter of the
 // rows per group: 4x the workgroups, which spreads one small image over more
 // CUs (a 256^2 reconstruction 199 -> 227 per second).
 template <int N>
 constexpr bool long_pow2() { return N >= 1024 && (N & (N - 1)) == 0; }
 
+// shapes 0-3 of the long power-of-two lengths (2 when twice the rows do
+// not fit a workgroup)
+template <int N, int... Rs>
+constexpr int long_row_shapes()
+{
+    using P16 = PlanE<16, N, Rs...>;
+    return (2 * P16::L * P16::T <= 1024 &&
+            sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 4 : 2;
+}
+
 template <int N, int... Rs>
 constexpr int row_shape_count()
 {
     using P16 = PlanE<16, N, Rs...>;
     if constexpr (long_pow2<N>()) {   // extra shapes for the long power-of-two lengths
-        return (2 * P16::L * P16::T <= 1024 &&
-                sizeof(cf) * (size_t) lds_elems(2 * P16::L * N) <= kLdsBudget) ? 4 : 2;
+        return long_row_shapes<N, Rs...>() + (P16::L >= 2 ? 1 : 0);   // + half the rows
     } else if constexpr (N <= 512 && P16::L >= 8) {
         return 2;   // short lines: also groups of a quarter of the rows (4x the workgroups)
     }
@@ -270,9 +282,13 @@ int launch_row_direct(int var, int mode, RowArgs a, hipStream_t s)
     if constexpr (long_pow2<N>() && ns > 1) {
         using P32 = PlanE<32, N, Rs...>;
         if (shape == 1) return launch_pf<P32, P32::L>(pf, mode, a, s);
-        if constexpr (ns == 4) {
+        constexpr int nb = long_row_shapes<N, Rs...>();
+        if constexpr (nb == 4) {
             if (shape == 2) return launch_pf<P16, 2 * P16::L>(pf, mode, a, s);
             if (shape == 3) return launch_pf<P16, P16::L, 1>(pf, mode, a, s);
+        }
+        if constexpr (ns > nb) {
+            if (shape == nb) return launch_pf<P16, P16::L / 2>(pf, mode, a, s);
         }
     } else if constexpr (ns > 1) {
         if (shape == 1) return launch_pf<P16, P16::L / 4>(pf, mode, a, s);
@@ -292,8 +308,10 @@ int row_variant_rows(int var)
     const int shape = var >= ns ? var - ns : var;
     if constexpr (long_pow2<N>() && ns > 1) {
         using P32 = PlanE<32, N, Rs...>;
+        constexpr int nb = long_row_shapes<N, Rs...>();
         if (shape == 1) return P32::L;
-        if (shape == 2) return 2 * P16::L;
+        if (nb == 4 && shape == 2) return 2 * P16::L;
+        if (ns > nb && shape == nb) return P16::L / 2;
     } else if constexpr (ns > 1) {
         if (shape == 1) return P16::L / 4;
     }
